@@ -1,0 +1,253 @@
+// Device-side data structures shared by the HIP kernels: per-order basis
+// tables in constant memory, the per-solve control block, kernel argument
+// blocks and the physics (Euler, linear advection) as device functions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dg_tables.h"
+
+namespace dgmi {
+
+// ---------------------------------------------------------------------------
+// Basis tables per order p, in the layouts the kernels index with
+// compile-time subscripts.  B[m][q] = phi_m at volume point q; Gx/Gy[m][q] =
+// w_q * dphi_m/dr, w_q * dphi_m/ds; S[f][m][j] = phi_m at point j of face f;
+// W1[j] Gauss weight; R1/R2/W volume points and weights (for projection).
+// ---------------------------------------------------------------------------
+template <typename R, int P>
+struct TabP {
+    static constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = n_quad_face(P);
+    R B[NP][NQ];
+    R Gx[NP][NQ];
+    R Gy[NP][NQ];
+    R S[3][NP][NQ1];
+    R W1[NQ1];
+    R R1[NQ], R2[NQ], W[NQ];
+    R V[NP][3];
+};
+
+template <typename R>
+struct TabAll {
+    TabP<R, 0> p0;
+    TabP<R, 1> p1;
+    TabP<R, 2> p2;
+    TabP<R, 3> p3;
+    TabP<R, 4> p4;
+    TabP<R, 5> p5;
+};
+
+extern __constant__ TabAll<double> c_tab64;
+extern __constant__ TabAll<float> c_tab32;
+
+template <typename R, int P>
+__device__ __forceinline__ const TabP<R, P>& tab();
+
+#define DGMI_TAB_SPEC(RT, SYM, PP)                                              \
+    template <>                                                                 \
+    __device__ __forceinline__ const TabP<RT, PP>& tab<RT, PP>() { return SYM.p##PP; }
+DGMI_TAB_SPEC(double, c_tab64, 0)
+DGMI_TAB_SPEC(double, c_tab64, 1)
+DGMI_TAB_SPEC(double, c_tab64, 2)
+DGMI_TAB_SPEC(double, c_tab64, 3)
+DGMI_TAB_SPEC(double, c_tab64, 4)
+DGMI_TAB_SPEC(double, c_tab64, 5)
+DGMI_TAB_SPEC(float, c_tab32, 0)
+DGMI_TAB_SPEC(float, c_tab32, 1)
+DGMI_TAB_SPEC(float, c_tab32, 2)
+DGMI_TAB_SPEC(float, c_tab32, 3)
+DGMI_TAB_SPEC(float, c_tab32, 4)
+DGMI_TAB_SPEC(float, c_tab32, 5)
+#undef DGMI_TAB_SPEC
+
+// ---------------------------------------------------------------------------
+// Per-solve control block in device memory: the time step is computed and
+// consumed on the device, so a step needs no host round trip.
+// ---------------------------------------------------------------------------
+struct Ctrl {
+    double dt;       // step size of the current step
+    double t;        // simulated time after the current step's advance
+    double t_end;    // end time (used when has_end)
+    double cfl;      // CFL number
+    double min_r;    // minimum inscribed-circle diameter of the mesh
+    double two_p1;   // 2p + 1
+    int64_t steps;   // steps taken
+    int active;      // the stages of the current step run
+    int done;        // t reached t_end
+    int has_end;     // clamp to t_end
+    int nparts;      // number of wave-speed partials to reduce
+    int pad[2];
+};
+
+struct PhysP {
+    double gamma;
+    double ax, ay;
+    int quirks;      // DG_QUIRK_* bits
+    int pad;
+};
+
+// Packed neighbour word per (element, face): bits 0..27 neighbour element,
+// bits 28..29 neighbour's local face, bit 30 set when this element is the
+// side's left element.
+constexpr int NB_ELEM_MASK = (1 << 28) - 1;
+__host__ __device__ inline int nb_pack(int e, int f, int is_left) { return e | (f << 28) | (is_left << 30); }
+
+enum StageKind {
+    ST_RHS = 0,   // out = dt * M^-1 R(u)
+    ST_RK4_1 = 1, // acc = k/6;      out = c + k/2
+    ST_RK4_2 = 2, // acc += k/3;     out = c + k/2
+    ST_RK4_3 = 3, // acc += k/3;     out = c + k
+    ST_RK4_4 = 4, // c = c + (acc + k/6); wave-speed partials
+    ST_RK2_1 = 5, // out = c + k/2
+    ST_RK2_2 = 6, // c = c + k; wave-speed partials
+};
+
+template <typename R>
+struct StageArgs {
+    const R* u;       // stage input [N][n_p][Kp]
+    const R* c;       // RK base state
+    R* acc;           // RK accumulator
+    R* out;           // next stage input, k, or c (in place) for the last stage
+    const R* geo_e;   // [5][Kp]: xr, yr, xs, ys, J
+    const R* geo_f;   // [9][Kp]: (nx, ny, len) for faces 0..2
+    const int* nbr;   // [3][Kp]
+    const Ctrl* ctrl;
+    double* lam_part; // per-tile maximum wave speed (last stage)
+    int K, Kp;
+    int kind;
+    int ntiles;       // tiles (workgroups) that carry elements
+    int xcd_remap;    // map consecutive tiles to one XCD
+    int pad;
+    PhysP phys;
+};
+
+// Workgroup -> tile mapping.  The dispatcher places workgroup b on XCD b % 8;
+// with remapping, XCD x receives the contiguous tile range
+// [x*per, (x+1)*per), so Morton neighbours share that XCD's L2.  The grid is
+// launched with 8*per workgroups; tiles >= ntiles exit.
+__device__ __forceinline__ int tile_of_block(int b, int nblocks, int remap) {
+    if (!remap) return b;
+    int per = nblocks >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Physics.  The formulas and their operation order are those of the
+// reference's euler.cu:61-168 (pressure, eval_c, eval_flux, eval_lambda) and
+// of the advection flux F = (ax u, ay u) with the LLF speed |a.n|.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R rdiv(R a, R b);
+template <>
+__device__ __forceinline__ double rdiv<double>(double a, double b) { return a / b; }
+template <>
+__device__ __forceinline__ float rdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+template <typename R>
+__device__ __forceinline__ R qsqrt(R x, int float_sqrt);
+template <>
+__device__ __forceinline__ double qsqrt<double>(double x, int fq) {
+    return fq ? (double)__builtin_sqrtf((float)x) : __builtin_sqrt(x);
+}
+template <>
+__device__ __forceinline__ float qsqrt<float>(float x, int) { return __builtin_sqrtf(x); }
+
+template <int N, typename R>
+__device__ __forceinline__ void flux(const PhysP& ph, const R* U, R* fx, R* fy) {
+    if constexpr (N == 1) {
+        fx[0] = (R)ph.ax * U[0];
+        fy[0] = (R)ph.ay * U[0];
+    } else {
+        const R rho = U[0], ru = U[1], rv = U[2], E = U[3];
+        const R g = (R)ph.gamma;
+        if constexpr (sizeof(R) == 8) {
+            R p = (g - R(1)) * (E - (ru * ru + rv * rv) / R(2) / rho);
+            fx[0] = ru;
+            fy[0] = rv;
+            fx[1] = ru * ru / rho + p;
+            fy[1] = ru * rv / rho;
+            fx[2] = ru * rv / rho;
+            fy[2] = rv * rv / rho + p;
+            fx[3] = ru * (E + p) / rho;
+            fy[3] = rv * (E + p) / rho;
+        } else {
+            R ir = __builtin_amdgcn_rcpf(rho);
+            R u = ru * ir, v = rv * ir;
+            R p = (g - R(1)) * (E - R(0.5) * (ru * u + rv * v));
+            fx[0] = ru;
+            fy[0] = rv;
+            fx[1] = ru * u + p;
+            fy[1] = rv * u;
+            fx[2] = ru * v;
+            fy[2] = rv * v + p;
+            fx[3] = (E + p) * u;
+            fy[3] = (E + p) * v;
+        }
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R sound_speed(const PhysP& ph, const R* U) {
+    const R rho = U[0], ru = U[1], rv = U[2], E = U[3];
+    const R g = (R)ph.gamma;
+    R p = (g - R(1)) * (E - (ru * ru + rv * rv) / R(2) / rho);
+    return qsqrt<R>(g * p / rho, ph.quirks & 1);
+}
+
+template <int N, typename R>
+__device__ __forceinline__ R llf_speed(const PhysP& ph, const R* UL, const R* UR, R nx, R ny) {
+    if constexpr (N == 1) {
+        return fabs((R)ph.ax * nx + (R)ph.ay * ny);
+    } else {
+        if constexpr (sizeof(R) == 8) {
+            R cl = sound_speed<R>(ph, UL), cr = sound_speed<R>(ph, UR);
+            R ul = UL[1] / UL[0], vl = UL[2] / UL[0];
+            R ur = UR[1] / UR[0], vr = UR[2] / UR[0];
+            R sl = nx * ul + ny * vl;
+            R sr = nx * ur + ny * vr;
+            R lm = (sl > R(0)) ? sl + cl : -sl + cl;
+            R rm = (sr > R(0)) ? sr + cr : -sr + cr;
+            return (fabs(lm) > fabs(rm)) ? fabs(lm) : fabs(rm);
+        } else {
+            const R g = (R)ph.gamma;
+            R il = __builtin_amdgcn_rcpf(UL[0]), ir = __builtin_amdgcn_rcpf(UR[0]);
+            R ul = UL[1] * il, vl = UL[2] * il, ur = UR[1] * ir, vr = UR[2] * ir;
+            R pl = (g - R(1)) * (UL[3] - R(0.5) * (UL[1] * ul + UL[2] * vl));
+            R pr = (g - R(1)) * (UR[3] - R(0.5) * (UR[1] * ur + UR[2] * vr));
+            R cl = __builtin_sqrtf(g * pl * il), cr = __builtin_sqrtf(g * pr * ir);
+            R lm = fabsf(nx * ul + ny * vl) + cl;
+            R rm = fabsf(nx * ur + ny * vr) + cr;
+            return fmaxf(lm, rm);
+        }
+    }
+}
+
+// CFL wave speed of one cell from its mean state (euler.cu:252-278), FP64.
+template <int N>
+__device__ __forceinline__ double cell_speed(const PhysP& ph, const double* U) {
+    if constexpr (N == 1) {
+        double a = ph.ax * ph.ax + ph.ay * ph.ay;
+        return (ph.quirks & 1) ? (double)__builtin_sqrtf((float)a) : __builtin_sqrt(a);
+    } else {
+        double c = sound_speed<double>(ph, U);
+        double m2 = U[1] * U[1] + U[2] * U[2];
+        double s = ((ph.quirks & 1) ? (double)__builtin_sqrtf((float)m2) : __builtin_sqrt(m2)) / U[0];
+        return (s > 0) ? s + c : -s + c;
+    }
+}
+
+// Maximum over a workgroup of blockDim.x lanes (multiple of 64): wavefront
+// butterfly with cross-lane shuffles, then one value per wave through LDS.
+__device__ __forceinline__ double block_max(double v, double* sh) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double r = sh[0];
+    for (int i = 1; i < nw; i++) r = fmax(r, sh[i]);
+    return r;
+}
+
+}  // namespace dgmi

@@ -1,0 +1,501 @@
+// Fused element kernels on the vector ALUs: one lane per element.
+//
+// One launch per RK stage does what the reference does with four
+// (eval_surface, eval_volume, eval_rhs, rk_tempstorage;
+// src/2d/conservation/time_integrator.cu:274-307):
+//   volume integral    (conserv_kernels.cu:766-811)
+//   LLF surface terms  (conserv_kernels.cu:628-758) evaluated for the three
+//                      faces of the element, each face flux computed in the
+//                      side's canonical left->right orientation so both
+//                      elements of a side obtain bit-identical fluxes
+//                      (race-free, no atomics, no face buffers)
+//   gather + M^-1      (time_integrator.cu:130-188, M = J*I)
+//   RK stage update    (time_integrator.cu:194-218, 445-464)
+//   CFL wave speeds    (euler.cu:252-278) of the new state on the last stage,
+//                      reduced per workgroup with wavefront shuffles.
+#include "dg_kernels.h"
+
+namespace dgmi {
+
+__constant__ TabAll<double> c_tab64;
+__constant__ TabAll<float> c_tab32;
+
+// phi on face f at point index j where j is a runtime value in [0, NQ1)
+// given as the compile-time pair (j, NQ1-1-j) selected by `rev`
+template <typename R, int P>
+__device__ __forceinline__ R face_phi(const TabP<R, P>& T, int f, int m, int j, bool rev) {
+    constexpr int NQ1 = P + 1;
+    return rev ? T.S[f][m][NQ1 - 1 - j] : T.S[f][m][j];
+}
+
+template <int N, int P, typename R>
+__global__ void __launch_bounds__(128) k_stage_valu(StageArgs<R> a) {
+    constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1;
+    const TabP<R, P>& T = tab<R, P>();
+    __shared__ double sh[8];
+    if (!a.ctrl->active) return;
+    const int tile = tile_of_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    if (tile >= a.ntiles) return;
+    const int e = tile * blockDim.x + threadIdx.x;
+    const bool valid = e < a.K;
+    const int ee = valid ? e : a.K - 1;
+    const int Kp = a.Kp;
+    const R dt = (R)a.ctrl->dt;
+
+    R u[N][NP];
+#pragma unroll
+    for (int v = 0; v < N; v++)
+#pragma unroll
+        for (int m = 0; m < NP; m++) u[v][m] = a.u[(size_t)(v * NP + m) * Kp + ee];
+
+    R tot[N][NP];
+#pragma unroll
+    for (int v = 0; v < N; v++)
+#pragma unroll
+        for (int m = 0; m < NP; m++) tot[v][m] = R(0);
+
+    // ---- volume: sum_q F(U_q) . (metric * grad phi_m) ----
+    {
+        const R xr = a.geo_e[ee], yr = a.geo_e[Kp + ee], xs = a.geo_e[2 * Kp + ee], ys = a.geo_e[3 * Kp + ee];
+#pragma unroll 2
+        for (int q = 0; q < NQ; q++) {
+            R U[N];
+#pragma unroll
+            for (int v = 0; v < N; v++) U[v] = R(0);
+#pragma unroll
+            for (int m = 0; m < NP; m++)
+#pragma unroll
+                for (int v = 0; v < N; v++) U[v] += u[v][m] * T.B[m][q];
+            R fx[N], fy[N];
+            flux<N, R>(a.phys, U, fx, fy);
+#pragma unroll
+            for (int m = 0; m < NP; m++) {
+                const R bgx = T.Gx[m][q], bgy = T.Gy[m][q];
+#pragma unroll
+                for (int v = 0; v < N; v++) tot[v][m] += fx[v] * (bgx * ys - bgy * yr) + fy[v] * (-bgx * xs + bgy * xr);
+            }
+        }
+    }
+
+    // ---- the three faces, added in face order as eval_rhs adds s1, s2, s3 ----
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const int w = a.nbr[f * Kp + ee];
+        const int nb = w & NB_ELEM_MASK;
+        const int fn = (w >> 28) & 3;
+        const bool left = (w >> 30) & 1;
+        const R nx = a.geo_f[(3 * f + 0) * Kp + ee];
+        const R ny = a.geo_f[(3 * f + 1) * Kp + ee];
+        const R len = a.geo_f[(3 * f + 2) * Kp + ee];
+        R FS[NQ1][N];
+#pragma unroll
+        for (int j = 0; j < NQ1; j++) {
+            // canonical point j of the side: the left element's trace at its
+            // point j, the right element's at NQ1-1-j
+            R Uo[N], Un[N];
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                Uo[v] = R(0);
+                Un[v] = R(0);
+            }
+#pragma unroll
+            for (int m = 0; m < NP; m++) {
+                const R so = face_phi<R, P>(T, f, m, j, !left);
+                const R s0 = face_phi<R, P>(T, 0, m, j, left);
+                const R s1 = face_phi<R, P>(T, 1, m, j, left);
+                const R s2 = face_phi<R, P>(T, 2, m, j, left);
+                const R sn = (fn == 0) ? s0 : ((fn == 1) ? s1 : s2);
+#pragma unroll
+                for (int v = 0; v < N; v++) {
+                    Uo[v] += u[v][m] * so;
+                    Un[v] += a.u[(size_t)(v * NP + m) * Kp + nb] * sn;
+                }
+            }
+            R UL[N], UR[N];
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                UL[v] = left ? Uo[v] : Un[v];
+                UR[v] = left ? Un[v] : Uo[v];
+            }
+            R fxl[N], fyl[N], fxr[N], fyr[N];
+            flux<N, R>(a.phys, UL, fxl, fyl);
+            flux<N, R>(a.phys, UR, fxr, fyr);
+            const R lam = llf_speed<N, R>(a.phys, UL, UR, nx, ny);
+#pragma unroll
+            for (int v = 0; v < N; v++)
+                FS[j][v] = T.W1[j] * (R(0.5) * ((fxl[v] + fxr[v]) * nx + (fyl[v] + fyr[v]) * ny + lam * (UL[v] - UR[v])));
+        }
+        const R coef = left ? -len / R(2) : len / R(2);
+#pragma unroll
+        for (int m = 0; m < NP; m++)
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                R s = R(0);
+#pragma unroll
+                for (int j = 0; j < NQ1; j++) s += FS[j][v] * face_phi<R, P>(T, f, m, j, !left);
+                tot[v][m] += coef * s;
+            }
+    }
+
+    // ---- inverse mass (J*I) and the RK stage update ----
+    const R J = a.geo_e[4 * Kp + ee];
+    const R cj = R(1) / J * dt;
+    const int kind = a.kind;
+    double lam = 0.0;
+    if (kind == ST_RHS) {
+        if (valid)
+#pragma unroll
+            for (int v = 0; v < N; v++)
+#pragma unroll
+                for (int m = 0; m < NP; m++) a.out[(size_t)(v * NP + m) * Kp + e] = cj * tot[v][m];
+    } else {
+        R c0[N];
+#pragma unroll
+        for (int v = 0; v < N; v++)
+#pragma unroll
+            for (int m = 0; m < NP; m++) {
+                const size_t ix = (size_t)(v * NP + m) * Kp + ee;
+                const R k = cj * tot[v][m];
+                const R c = (kind == ST_RK4_1 || kind == ST_RK2_1) ? u[v][m] : a.c[ix];
+                R o;
+                if (kind == ST_RK4_1) {
+                    if (valid) a.acc[ix] = k / R(6);
+                    o = c + R(0.5) * k;
+                } else if (kind == ST_RK4_2 || kind == ST_RK4_3) {
+                    if (valid) a.acc[ix] = a.acc[ix] + k / R(3);
+                    o = (kind == ST_RK4_2) ? c + R(0.5) * k : c + R(1.0) * k;
+                } else if (kind == ST_RK4_4) {
+                    o = c + (a.acc[ix] + k / R(6));
+                } else if (kind == ST_RK2_1) {
+                    o = c + R(0.5) * k;
+                } else {
+                    o = c + k;
+                }
+                if (valid) a.out[ix] = o;
+                if (m == 0) c0[v] = o;
+            }
+        if (kind == ST_RK4_4 || kind == ST_RK2_2) {
+            double U[4];
+            const double b0 = tab<double, P>().B[0][0];
+#pragma unroll
+            for (int v = 0; v < N; v++) U[v] = (double)c0[v] * b0;
+            lam = valid ? cell_speed<N>(a.phys, U) : 0.0;
+        }
+    }
+    if (kind == ST_RK4_4 || kind == ST_RK2_2) {
+        const double mx = block_max(lam, sh);
+        if (threadIdx.x == 0) a.lam_part[tile] = mx;
+    }
+}
+
+// Wave-speed partials of a state without stepping it (first step of a run,
+// dg_cfl_dt).  Same tile geometry as the stage kernel it precedes.
+template <int N, typename R>
+__global__ void __launch_bounds__(256) k_lambda(const R* u, int K, int Kp, int NP, int tile_elems, int ntiles,
+                                               int xcd_remap, double b0, PhysP ph, double* lam_part) {
+    __shared__ double sh[8];
+    const int tile = tile_of_block(blockIdx.x, gridDim.x, xcd_remap);
+    if (tile >= ntiles) return;
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < tile_elems; i += blockDim.x) {
+        const int e = tile * tile_elems + i;
+        if (e < K) {
+            double U[4];
+#pragma unroll
+            for (int v = 0; v < N; v++) U[v] = (double)u[(size_t)(v * NP) * Kp + e] * b0;
+            mx = fmax(mx, cell_speed<N>(ph, U));
+        }
+    }
+    mx = block_max(mx, sh);
+    if (threadIdx.x == 0) lam_part[tile] = mx;
+}
+
+// dt = cfl * min_r / max lambda / (2p+1), clamped to t_end
+// (time_integrator.cu:254-270); one workgroup.
+__global__ void __launch_bounds__(256) k_dt(Ctrl* ctrl, const double* lam_part) {
+    __shared__ double sh[8];
+    const int n = ctrl->nparts;
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = fmax(mx, lam_part[i]);
+    mx = block_max(mx, sh);
+    if (threadIdx.x == 0) {
+        if (ctrl->done) {
+            ctrl->active = 0;
+            return;
+        }
+        double t = ctrl->t;
+        if (ctrl->has_end && !(t < ctrl->t_end)) {
+            ctrl->active = 0;
+            ctrl->done = 1;
+            ctrl->dt = 0.0;
+            return;
+        }
+        double dt = ctrl->cfl * ctrl->min_r / mx / ctrl->two_p1;
+        if (ctrl->has_end && t + dt > ctrl->t_end) {
+            dt = ctrl->t_end - t;
+            t = ctrl->t_end;
+        } else {
+            t += dt;
+        }
+        ctrl->dt = dt;
+        ctrl->t = t;
+        ctrl->steps += 1;
+        ctrl->active = 1;
+    }
+}
+
+// host-chosen step: activate the stages with the given dt
+__global__ void k_set_dt(Ctrl* ctrl, double dt, int advance) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctrl->dt = dt;
+        ctrl->active = 1;
+        if (advance) {
+            ctrl->t += dt;
+            ctrl->steps += 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Initial conditions: c_i = sum_j w_j U0(x_j) phi_i(x_j) in FP64
+// (conserv_kernels.cu:167-187, euler.cu:31-59), stored in the bulk precision.
+// ---------------------------------------------------------------------------
+__device__ inline double wrap_d(double d, double L) { return (L > 0) ? d - L * rint(d / L) : d; }
+
+__device__ void ic_point(const IcP& ic, double x, double y, double* U) {
+    if (ic.kind == 0) {
+        for (int n = 0; n < 4; n++) U[n] = ic.constant[n];
+        return;
+    }
+    if (ic.kind == 1) {
+        double dx = wrap_d(x - ic.gx, ic.Lx), dy = wrap_d(y - ic.gy, ic.Ly);
+        U[0] = exp(-ic.gk * (dx * dx + dy * dy));
+        return;
+    }
+    const double PI = 3.14159265358979323846;
+    const double g = ic.gamma;
+    double du = 0, dv = 0, dT = 0;
+    for (int k = 0; k < ic.nv; k++) {
+        double dx = wrap_d(x - ic.vx[k], ic.Lx), dy = wrap_d(y - ic.vy[k], ic.Ly);
+        double r2 = dx * dx + dy * dy;
+        double ex = exp(0.5 * (1. - r2));
+        double beta = ic.beta[k];
+        du += -beta / (2. * PI) * dy * ex;
+        dv += beta / (2. * PI) * dx * ex;
+        dT += -(g - 1.) * beta * beta / (8. * g * PI * PI) * ex * ex;
+    }
+    double T_inf = ic.p_inf / ic.rho_inf;
+    double T = T_inf + dT;
+    double rho = ic.rho_inf * pow(T / T_inf, 1. / (g - 1.));
+    double p = ic.p_inf * pow(rho / ic.rho_inf, g);
+    double uu = ic.u_inf + du, vv = ic.v_inf + dv;
+    U[0] = rho;
+    U[1] = rho * uu;
+    U[2] = rho * vv;
+    U[3] = p / (g - 1.) + 0.5 * rho * (uu * uu + vv * vv);
+}
+
+template <int N, int P, typename R>
+__global__ void __launch_bounds__(128) k_project_ic(R* c, const double* V, int K, int Kp, IcP ic) {
+    constexpr int NP = n_modes(P), NQ = n_quad_vol(P);
+    const TabP<double, P>& T = tab<double, P>();
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= K) return;
+    const double v1x = V[e], v1y = V[Kp + e], v2x = V[2 * Kp + e], v2y = V[3 * Kp + e];
+    const double v3x = V[4 * Kp + e], v3y = V[5 * Kp + e];
+    double u0[NQ][N];
+    for (int j = 0; j < NQ; j++) {
+        const double r1 = T.R1[j], r2 = T.R2[j];
+        const double x = r1 * v2x + r2 * v3x + (1 - r1 - r2) * v1x;
+        const double y = r1 * v2y + r2 * v3y + (1 - r1 - r2) * v1y;
+        double U[4];
+        ic_point(ic, x, y, U);
+        for (int n = 0; n < N; n++) u0[j][n] = U[n];
+    }
+    for (int i = 0; i < NP; i++) {
+        double U[N];
+        for (int n = 0; n < N; n++) U[n] = 0.;
+        for (int j = 0; j < NQ; j++)
+            for (int n = 0; n < N; n++) U[n] += T.W[j] * u0[j][n] * T.B[i][j];
+        for (int n = 0; n < N; n++) c[(size_t)(n * NP + i) * Kp + e] = (R)U[n];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 audit partials per workgroup: mass of each variable, minimum cell-mean
+// density and pressure, count of non-finite / non-positive cells.
+// part[blk*8 + 0..3] mass, +4 min rho, +5 min p, +6 bad count.
+// ---------------------------------------------------------------------------
+template <int N, typename R>
+__global__ void __launch_bounds__(256) k_audit(const R* c, const double* Jd, int K, int Kp, int NP, double b0,
+                                              double gamma, double* part) {
+    __shared__ double sm[256];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double mass[4] = {0, 0, 0, 0}, rho = 1e300, pr = 1e300, bad = 0;
+    if (e < K) {
+        double U[4];
+        const double J = Jd[e];
+        for (int v = 0; v < N; v++) {
+            double c0 = (double)c[(size_t)(v * NP) * Kp + e];
+            mass[v] = J * c0 * b0 * 0.5;
+            U[v] = c0 * b0;
+        }
+        rho = U[0];
+        bool ok = isfinite(U[0]);
+        if (N == 4) {
+            for (int v = 1; v < 4; v++) ok = ok && isfinite(U[v]);
+            pr = (gamma - 1.) * (U[3] - (U[1] * U[1] + U[2] * U[2]) / 2. / U[0]);
+            ok = ok && U[0] > 0 && pr > 0;
+        }
+        bad = ok ? 0 : 1;
+    }
+    // fixed-shape tree reduction: deterministic
+    double vals[7] = {mass[0], mass[1], mass[2], mass[3], rho, pr, bad};
+    for (int k = 0; k < 7; k++) {
+        sm[threadIdx.x] = vals[k];
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                double x = sm[threadIdx.x], y = sm[threadIdx.x + s];
+                sm[threadIdx.x] = (k == 4 || k == 5) ? fmin(x, y) : x + y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[blockIdx.x * 8 + k] = sm[0];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int N, typename R>
+static hipError_t launch_stage_valu_np(int p, const StageArgs<R>& a, dim3 grid, hipStream_t st) {
+    switch (p) {
+        case 0: hipLaunchKernelGGL((k_stage_valu<N, 0, R>), grid, dim3(128), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_stage_valu<N, 1, R>), grid, dim3(128), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_stage_valu<N, 2, R>), grid, dim3(128), 0, st, a); break;
+        case 3: hipLaunchKernelGGL((k_stage_valu<N, 3, R>), grid, dim3(128), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_stage_valu<N, 4, R>), grid, dim3(128), 0, st, a); break;
+        case 5: hipLaunchKernelGGL((k_stage_valu<N, 5, R>), grid, dim3(128), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_stage_valu(int N, int p, const StageArgs<R>& a, dim3 grid, hipStream_t st) {
+    if (N == 1) return launch_stage_valu_np<1, R>(p, a, grid, st);
+    if (N == 4) return launch_stage_valu_np<4, R>(p, a, grid, st);
+    return hipErrorInvalidValue;
+}
+template hipError_t launch_stage_valu<float>(int, int, const StageArgs<float>&, dim3, hipStream_t);
+template hipError_t launch_stage_valu<double>(int, int, const StageArgs<double>&, dim3, hipStream_t);
+
+template <typename R>
+hipError_t launch_lambda(int N, const R* u, int K, int Kp, int NP, int tile_elems, int ntiles, int xcd_remap,
+                         double b0, const PhysP& ph, double* lam_part, dim3 grid, hipStream_t st) {
+    if (N == 1)
+        hipLaunchKernelGGL((k_lambda<1, R>), grid, dim3(256), 0, st, u, K, Kp, NP, tile_elems, ntiles, xcd_remap, b0, ph,
+                           lam_part);
+    else
+        hipLaunchKernelGGL((k_lambda<4, R>), grid, dim3(256), 0, st, u, K, Kp, NP, tile_elems, ntiles, xcd_remap, b0, ph,
+                           lam_part);
+    return hipGetLastError();
+}
+template hipError_t launch_lambda<float>(int, const float*, int, int, int, int, int, int, double, const PhysP&,
+                                         double*, dim3, hipStream_t);
+template hipError_t launch_lambda<double>(int, const double*, int, int, int, int, int, int, double, const PhysP&,
+                                          double*, dim3, hipStream_t);
+
+hipError_t launch_dt(Ctrl* ctrl, const double* lam_part, hipStream_t st) {
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, st, ctrl, lam_part);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_dt(Ctrl* ctrl, double dt, int advance, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, st, ctrl, dt, advance);
+    return hipGetLastError();
+}
+
+template <int N, typename R>
+static hipError_t launch_ic_np(int p, R* c, const double* V, int K, int Kp, const IcP& ic, hipStream_t st) {
+    dim3 grid((K + 127) / 128);
+    switch (p) {
+        case 0: hipLaunchKernelGGL((k_project_ic<N, 0, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
+        case 1: hipLaunchKernelGGL((k_project_ic<N, 1, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
+        case 2: hipLaunchKernelGGL((k_project_ic<N, 2, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
+        case 3: hipLaunchKernelGGL((k_project_ic<N, 3, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
+        case 4: hipLaunchKernelGGL((k_project_ic<N, 4, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
+        case 5: hipLaunchKernelGGL((k_project_ic<N, 5, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_project_ic(int N, int p, R* c, const double* V, int K, int Kp, const IcP& ic, hipStream_t st) {
+    if (N == 1) return launch_ic_np<1, R>(p, c, V, K, Kp, ic, st);
+    return launch_ic_np<4, R>(p, c, V, K, Kp, ic, st);
+}
+template hipError_t launch_project_ic<float>(int, int, float*, const double*, int, int, const IcP&, hipStream_t);
+template hipError_t launch_project_ic<double>(int, int, double*, const double*, int, int, const IcP&, hipStream_t);
+
+template <typename R>
+hipError_t launch_audit(int N, const R* c, const double* Jd, int K, int Kp, int NP, double b0, double gamma,
+                        double* part, int nblocks, hipStream_t st) {
+    if (N == 1)
+        hipLaunchKernelGGL((k_audit<1, R>), dim3(nblocks), dim3(256), 0, st, c, Jd, K, Kp, NP, b0, gamma, part);
+    else
+        hipLaunchKernelGGL((k_audit<4, R>), dim3(nblocks), dim3(256), 0, st, c, Jd, K, Kp, NP, b0, gamma, part);
+    return hipGetLastError();
+}
+template hipError_t launch_audit<float>(int, const float*, const double*, int, int, int, double, double, double*,
+                                        int, hipStream_t);
+template hipError_t launch_audit<double>(int, const double*, const double*, int, int, int, double, double, double*,
+                                         int, hipStream_t);
+
+// Fill the constant-memory tables of every order from the FP64 host tables.
+template <typename R, int P>
+static void fill_tab(TabP<R, P>& d) {
+    Tables t = build_tables(P);
+    constexpr int NP = TabP<R, P>::NP, NQ = TabP<R, P>::NQ, NQ1 = TabP<R, P>::NQ1;
+    for (int m = 0; m < NP; m++) {
+        for (int q = 0; q < NQ; q++) {
+            d.B[m][q] = (R)t.basis[m * NQ + q];
+            d.Gx[m][q] = (R)t.gx[m * NQ + q];
+            d.Gy[m][q] = (R)t.gy[m * NQ + q];
+        }
+        for (int f = 0; f < 3; f++)
+            for (int j = 0; j < NQ1; j++) d.S[f][m][j] = (R)t.side[(f * NP + m) * NQ1 + j];
+        for (int v = 0; v < 3; v++) d.V[m][v] = (R)t.vertex[m * 3 + v];
+    }
+    for (int j = 0; j < NQ1; j++) d.W1[j] = (R)t.w1[j];
+    for (int q = 0; q < NQ; q++) {
+        d.R1[q] = (R)t.r1[q];
+        d.R2[q] = (R)t.r2[q];
+        d.W[q] = (R)t.w[q];
+    }
+}
+
+template <typename R>
+static void fill_all(TabAll<R>& a) {
+    fill_tab<R, 0>(a.p0);
+    fill_tab<R, 1>(a.p1);
+    fill_tab<R, 2>(a.p2);
+    fill_tab<R, 3>(a.p3);
+    fill_tab<R, 4>(a.p4);
+    fill_tab<R, 5>(a.p5);
+}
+
+hipError_t upload_tables() {
+    static TabAll<double> h64;
+    static TabAll<float> h32;
+    fill_all<double>(h64);
+    fill_all<float>(h32);
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab64), &h64, sizeof(h64));
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &h32, sizeof(h32));
+}
+
+}  // namespace dgmi

@@ -1,0 +1,217 @@
+#include "dg_mesh.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+namespace dgmi {
+
+uint64_t splitmix64(uint64_t& state) {
+    uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double uniform01(uint64_t& st) { return (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0); }
+
+// .pmsh, conservation dialect (conserv.cu:134-191): num_elem, then one line per
+// element "V1x V1y V2x V2y V3x V3y s1 s2 s3", num_sides, then one line per side
+// "x1 y1 x2 y2 left_elem right_elem left_side_number right_side_number".
+std::string read_pmsh(const char* path, Mesh& m) {
+    FILE* f = fopen(path, "r");
+    if (!f) return std::string("cannot open mesh file ") + path;
+    char line[1024];
+    auto fail = [&](const std::string& why) {
+        fclose(f);
+        return why;
+    };
+    if (!fgets(line, sizeof line, f) || sscanf(line, "%d", &m.K) != 1 || m.K <= 0)
+        return fail("bad element count line");
+    const int K = m.K;
+    m.V1x.resize(K); m.V1y.resize(K); m.V2x.resize(K); m.V2y.resize(K); m.V3x.resize(K); m.V3y.resize(K);
+    m.elem_s1.resize(K); m.elem_s2.resize(K); m.elem_s3.resize(K);
+    for (int i = 0; i < K; i++) {
+        if (!fgets(line, sizeof line, f)) return fail("unexpected end of file in element list");
+        int items = sscanf(line, "%lf %lf %lf %lf %lf %lf %d %d %d", &m.V1x[i], &m.V1y[i], &m.V2x[i], &m.V2y[i],
+                           &m.V3x[i], &m.V3y[i], &m.elem_s1[i], &m.elem_s2[i], &m.elem_s3[i]);
+        if (items != 9) return fail("element line " + std::to_string(i) + ": expected 9 items, got " + std::to_string(items));
+    }
+    if (!fgets(line, sizeof line, f) || sscanf(line, "%d", &m.S) != 1 || m.S <= 0) return fail("bad side count line");
+    const int S = m.S;
+    m.sx1.resize(S); m.sy1.resize(S); m.sx2.resize(S); m.sy2.resize(S);
+    m.left_elem.resize(S); m.right_elem.resize(S); m.left_side.resize(S); m.right_side.resize(S);
+    for (int i = 0; i < S; i++) {
+        if (!fgets(line, sizeof line, f)) return fail("unexpected end of file in side list");
+        int items = sscanf(line, "%lf %lf %lf %lf %d %d %d %d", &m.sx1[i], &m.sy1[i], &m.sx2[i], &m.sy2[i],
+                           &m.left_elem[i], &m.right_elem[i], &m.left_side[i], &m.right_side[i]);
+        if (items != 8) return fail("side line " + std::to_string(i) + ": expected 8 items, got " + std::to_string(items));
+    }
+    fclose(f);
+    m.period_x = m.period_y = 0;
+    return validate(m);
+}
+
+std::string write_pmsh(const char* path, const Mesh& m) {
+    FILE* f = fopen(path, "w");
+    if (!f) return std::string("cannot open ") + path + " for writing";
+    fprintf(f, "%d\n", m.K);
+    for (int i = 0; i < m.K; i++)
+        fprintf(f, "%.17g %.17g %.17g %.17g %.17g %.17g %d %d %d\n", m.V1x[i], m.V1y[i], m.V2x[i], m.V2y[i], m.V3x[i],
+                m.V3y[i], m.elem_s1[i], m.elem_s2[i], m.elem_s3[i]);
+    fprintf(f, "%d\n", m.S);
+    for (int i = 0; i < m.S; i++)
+        fprintf(f, "%.17g %.17g %.17g %.17g %d %d %d %d\n", m.sx1[i], m.sy1[i], m.sx2[i], m.sy2[i], m.left_elem[i],
+                m.right_elem[i], m.left_side[i], m.right_side[i]);
+    int bad = ferror(f);
+    fclose(f);
+    return bad ? std::string("write error on ") + path : std::string();
+}
+
+static uint32_t spread16(uint32_t v) {
+    v &= 0xFFFF;
+    v = (v | (v << 8)) & 0x00FF00FF;
+    v = (v | (v << 4)) & 0x0F0F0F0F;
+    v = (v | (v << 2)) & 0x33333333;
+    v = (v | (v << 1)) & 0x55555555;
+    return v;
+}
+
+static uint32_t morton2(uint32_t i, uint32_t j) { return spread16(i) | (spread16(j) << 1); }
+
+std::string make_periodic_jittered(int nx, int ny, double x0, double x1, double y0, double y1, double jitter,
+                                   uint64_t seed, Mesh& m) {
+    if (nx < 1 || ny < 1 || nx > 65535 || ny > 65535) return "grid size out of range";
+    if (!(x1 > x0) || !(y1 > y0)) return "empty domain";
+    if (!(jitter >= 0) || jitter >= 0.25) return "jitter must lie in [0, 0.25)";
+    const double hx = (x1 - x0) / nx, hy = (y1 - y0) / ny;
+    // lattice (nx+1) x (ny+1); index nx / ny is the periodic copy of 0
+    std::vector<double> vx((size_t)(nx + 1) * (ny + 1)), vy(vx.size());
+    uint64_t st = seed;
+    auto vid = [&](int i, int j) { return (size_t)j * (nx + 1) + i; };
+    for (int j = 0; j <= ny; j++)
+        for (int i = 0; i <= nx; i++) {
+            vx[vid(i, j)] = (i == nx) ? x1 : x0 + i * hx;
+            vy[vid(i, j)] = (j == ny) ? y1 : y0 + j * hy;
+        }
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            double ux = uniform01(st), uy = uniform01(st);
+            if (i > 0 && j > 0) {
+                vx[vid(i, j)] += (2.0 * ux - 1.0) * jitter * hx;
+                vy[vid(i, j)] += (2.0 * uy - 1.0) * jitter * hy;
+            }
+        }
+    // cells in Morton order
+    const int C = nx * ny;
+    std::vector<int> cells(C);
+    std::iota(cells.begin(), cells.end(), 0);
+    std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) {
+        return morton2(a % nx, a / nx) < morton2(b % nx, b / nx);
+    });
+    std::vector<int> cell_rank(C);
+    for (int r = 0; r < C; r++) cell_rank[cells[r]] = r;
+    const int K = 2 * C;
+    m = Mesh();
+    m.K = K;
+    m.V1x.resize(K); m.V1y.resize(K); m.V2x.resize(K); m.V2y.resize(K); m.V3x.resize(K); m.V3y.resize(K);
+    for (int r = 0; r < C; r++) {
+        int i = cells[r] % nx, j = cells[r] / nx;
+        size_t a = vid(i, j), b = vid(i + 1, j), c = vid(i + 1, j + 1), d = vid(i, j + 1);
+        int L = 2 * r, U = 2 * r + 1;
+        m.V1x[L] = vx[a]; m.V1y[L] = vy[a]; m.V2x[L] = vx[b]; m.V2y[L] = vy[b]; m.V3x[L] = vx[c]; m.V3y[L] = vy[c];
+        m.V1x[U] = vx[a]; m.V1y[U] = vy[a]; m.V2x[U] = vx[c]; m.V2y[U] = vy[c]; m.V3x[U] = vx[d]; m.V3y[U] = vy[d];
+    }
+    // neighbour (element, face) of every (element, face)
+    auto elem_of = [&](int i, int j, int upper) {
+        i = ((i % nx) + nx) % nx;
+        j = ((j % ny) + ny) % ny;
+        return 2 * cell_rank[j * nx + i] + upper;
+    };
+    std::vector<int> nb_e(3 * (size_t)K), nb_f(3 * (size_t)K);
+    for (int r = 0; r < C; r++) {
+        int i = cells[r] % nx, j = cells[r] / nx;
+        int L = 2 * r, U = 2 * r + 1;
+        // lower: f0 = bottom edge (shared with the upper triangle of the cell below, its f1)
+        //        f1 = right edge (upper triangle of the cell to the right, its f2)
+        //        f2 = diagonal (upper triangle of the same cell, its f0)
+        nb_e[3 * L + 0] = elem_of(i, j - 1, 1); nb_f[3 * L + 0] = 1;
+        nb_e[3 * L + 1] = elem_of(i + 1, j, 1); nb_f[3 * L + 1] = 2;
+        nb_e[3 * L + 2] = U;                    nb_f[3 * L + 2] = 0;
+        nb_e[3 * U + 0] = L;                    nb_f[3 * U + 0] = 2;
+        nb_e[3 * U + 1] = elem_of(i, j + 1, 0); nb_f[3 * U + 1] = 0;
+        nb_e[3 * U + 2] = elem_of(i - 1, j, 0); nb_f[3 * U + 2] = 1;
+    }
+    std::vector<int> side_of(3 * (size_t)K, -1);
+    m.elem_s1.resize(K); m.elem_s2.resize(K); m.elem_s3.resize(K);
+    for (int e = 0; e < K; e++)
+        for (int f = 0; f < 3; f++) {
+            if (side_of[3 * e + f] >= 0) continue;
+            int s = m.S++;
+            int ne = nb_e[3 * e + f], nf = nb_f[3 * e + f];
+            side_of[3 * e + f] = s;
+            side_of[3 * ne + nf] = s;
+            double ax_, ay_, bx_, by_;
+            if (f == 0) { ax_ = m.V1x[e]; ay_ = m.V1y[e]; bx_ = m.V2x[e]; by_ = m.V2y[e]; }
+            else if (f == 1) { ax_ = m.V2x[e]; ay_ = m.V2y[e]; bx_ = m.V3x[e]; by_ = m.V3y[e]; }
+            else { ax_ = m.V3x[e]; ay_ = m.V3y[e]; bx_ = m.V1x[e]; by_ = m.V1y[e]; }
+            m.sx1.push_back(ax_); m.sy1.push_back(ay_); m.sx2.push_back(bx_); m.sy2.push_back(by_);
+            m.left_elem.push_back(e); m.right_elem.push_back(ne);
+            m.left_side.push_back(f); m.right_side.push_back(nf);
+        }
+    for (int e = 0; e < K; e++) {
+        m.elem_s1[e] = side_of[3 * e + 0];
+        m.elem_s2[e] = side_of[3 * e + 1];
+        m.elem_s3[e] = side_of[3 * e + 2];
+    }
+    m.period_x = x1 - x0;
+    m.period_y = y1 - y0;
+    m.x0 = x0;
+    m.y0 = y0;
+    m.ordered = true;
+    return validate(m);
+}
+
+std::string validate(const Mesh& m) {
+    if (m.K <= 0 || m.S <= 0) return "empty mesh";
+    const int* es[3] = {m.elem_s1.data(), m.elem_s2.data(), m.elem_s3.data()};
+    for (int e = 0; e < m.K; e++)
+        for (int f = 0; f < 3; f++) {
+            int s = es[f][e];
+            if (s < 0 || s >= m.S) return "element " + std::to_string(e) + " face " + std::to_string(f) + " has side id out of range";
+            bool l = m.left_elem[s] == e && m.left_side[s] == f;
+            bool r = m.right_elem[s] == e && m.right_side[s] == f;
+            if (!l && !r) return "element " + std::to_string(e) + " face " + std::to_string(f) + " is not referenced by its side";
+        }
+    for (int s = 0; s < m.S; s++) {
+        int l = m.left_elem[s], r = m.right_elem[s];
+        if (l < 0 || l >= m.K) return "side " + std::to_string(s) + " has left element out of range";
+        if (r >= m.K || r < -3) return "side " + std::to_string(s) + " has right element out of range";
+        if (m.left_side[s] < 0 || m.left_side[s] > 2) return "side " + std::to_string(s) + " bad left side number";
+        if (r >= 0 && (m.right_side[s] < 0 || m.right_side[s] > 2)) return "side " + std::to_string(s) + " bad right side number";
+    }
+    return std::string();
+}
+
+std::vector<int> morton_order(const Mesh& m) {
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+    std::vector<double> cx(m.K), cy(m.K);
+    for (int e = 0; e < m.K; e++) {
+        cx[e] = (m.V1x[e] + m.V2x[e] + m.V3x[e]) / 3.0;
+        cy[e] = (m.V1y[e] + m.V2y[e] + m.V3y[e]) / 3.0;
+        xmin = std::min(xmin, cx[e]); xmax = std::max(xmax, cx[e]);
+        ymin = std::min(ymin, cy[e]); ymax = std::max(ymax, cy[e]);
+    }
+    double sx = (xmax > xmin) ? 65535.0 / (xmax - xmin) : 0, sy = (ymax > ymin) ? 65535.0 / (ymax - ymin) : 0;
+    std::vector<uint32_t> key(m.K);
+    for (int e = 0; e < m.K; e++)
+        key[e] = morton2((uint32_t)((cx[e] - xmin) * sx), (uint32_t)((cy[e] - ymin) * sy));
+    std::vector<int> perm(m.K);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    return perm;
+}
+
+}  // namespace dgmi

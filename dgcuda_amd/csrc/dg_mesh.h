@@ -1,0 +1,54 @@
+// Triangle mesh in the reference's read_mesh layout plus the element-centric
+// Morton-ordered connectivity the HIP kernels use.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dgmi {
+
+// Arrays exactly as the reference's read_mesh returns them
+// (src/2d/conservation/conserv.cu:118-192): vertices V1..V3 per element,
+// the global side id of each local face (face 0 = V1->V2, 1 = V2->V3,
+// 2 = V3->V1), and per side its end points in the left element's
+// orientation, left/right element and the local face number on each side.
+struct Mesh {
+    int K = 0, S = 0;
+    std::vector<double> V1x, V1y, V2x, V2y, V3x, V3y;
+    std::vector<int> elem_s1, elem_s2, elem_s3;
+    std::vector<double> sx1, sy1, sx2, sy2;
+    std::vector<int> left_elem, right_elem, left_side, right_side;
+    // periodic extent of the domain (0 when the mesh is not periodic); used
+    // by the initial conditions to measure distances to the nearest image
+    double period_x = 0, period_y = 0;
+    double x0 = 0, y0 = 0;
+    // true when the element order is already Morton (generated meshes)
+    bool ordered = false;
+};
+
+// Returns an empty string on success, otherwise the reason.
+std::string read_pmsh(const char* path, Mesh& m);
+std::string write_pmsh(const char* path, const Mesh& m);
+
+// n_x * n_y squares on [x0,x1] x [y0,y1], periodic in both directions, each
+// split along its (i,j)-(i+1,j+1) diagonal into two counter-clockwise
+// triangles.  Lattice vertices off the periodic boundary are moved by
+// independent uniform offsets in [-jitter*h, jitter*h] per coordinate, drawn
+// from splitmix64(seed) in row-major vertex order (two draws per vertex, x
+// then y, for every vertex 0 <= i < n_x, 0 <= j < n_y; draws for boundary
+// vertices are made and discarded).  Cells are enumerated in Morton order of
+// (i, j); cell c owns elements 2c (lower) and 2c+1 (upper).  Sides are
+// numbered in order of first appearance over (element, local face), and the
+// element that first meets a side is its left element.
+std::string make_periodic_jittered(int nx, int ny, double x0, double x1, double y0, double y1, double jitter,
+                                   uint64_t seed, Mesh& m);
+
+// Checks that the side/element cross references are consistent.
+std::string validate(const Mesh& m);
+
+// Element permutation by the Morton code of the centroid (new -> old).
+std::vector<int> morton_order(const Mesh& m);
+
+uint64_t splitmix64(uint64_t& state);
+
+}  // namespace dgmi

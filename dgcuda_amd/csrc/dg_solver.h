@@ -1,0 +1,74 @@
+// One DG solve resident on the GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dgmi.h"
+#include "dg_device.h"
+#include "dg_geometry.h"
+#include "dg_kernels.h"
+#include "dg_mesh.h"
+
+namespace dgmi {
+
+struct Status {
+    int code = DG_OK;
+    std::string msg;
+    static Status ok() { return Status(); }
+    static Status err(int c, std::string m) {
+        Status s;
+        s.code = c;
+        s.msg = std::move(m);
+        return s;
+    }
+    explicit operator bool() const { return code == DG_OK; }
+};
+
+Status hip_status(hipError_t e, const char* what);
+Status ensure_device_ready();
+
+PhysP make_phys(const dg_problem_desc& d);
+IcP make_ic(const dg_problem_desc& d, double Lx, double Ly);
+Status check_problem(const dg_problem_desc& d, int p, int precision);
+
+// Kernel selection and launch shape of the stage kernels.
+struct LaunchShape {
+    int kind = 1;        // 1 = VALU (lane per element), 2 = matrix cores
+    int tile_elems = 0;  // elements per workgroup
+    int ntiles = 0;      // workgroups carrying elements
+    int grid = 0;        // workgroups launched (multiple of 8 when remapped)
+    int xcd_remap = 1;
+};
+
+class Solver {
+   public:
+    virtual ~Solver() {}
+    static Status create(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out);
+
+    int K = 0, S = 0, Kp = 0, p = 0, NP = 0, NQ = 0, NQ1 = 0, N = 0, precision = DG_FP64;
+    double min_r = 0;
+    double t = 0;
+    int64_t steps = 0;
+    dg_problem_desc prob{};
+
+    virtual Status project_ic() = 0;
+    virtual Status set_state(const double* c_ref) = 0;
+    virtual Status get_state(double* c_ref) = 0;
+    virtual Status rhs(const double* u_ref, double* k_ref, double dt) = 0;
+    virtual Status rk_step(double dt, int scheme) = 0;
+    virtual Status cfl_dt(double cfl, double* dt) = 0;
+    virtual Status integrate(double t_end, double cfl, int scheme, int64_t* nsteps) = 0;
+    virtual Status run_steps(int nsteps, double cfl, int scheme, bool use_graph) = 0;
+    virtual Status audit(dg_audit_out* out) = 0;
+    virtual Status eval_vertices(int var, double* uv) = 0;
+    virtual Status time_stage_kernels(int nsteps, double cfl, double* out) = 0;
+    virtual Status set_kernel(int kind) = 0;
+    virtual int kernel_kind() const = 0;
+    virtual int64_t device_bytes() const = 0;
+    virtual Status sync() = 0;
+};
+
+}  // namespace dgmi

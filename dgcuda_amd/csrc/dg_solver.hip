@@ -1,0 +1,583 @@
+// Host driver of one solve: device buffers, stage sequencing, device-side
+// CFL time step, hipGraph replay of a step, state I/O and audits.
+//
+// The reference's time loop (src/2d/conservation/time_integrator.cu:221-434)
+// launches four kernels per stage and copies every element's wave speed to
+// the host each step to pick dt.  Here a step is one small dt kernel plus one
+// fused launch per stage; dt never leaves the device, and the host only
+// synchronises once per chunk of steps to test for t >= t_end.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "dg_solver.h"
+
+namespace dgmi {
+
+Status hip_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return Status::ok();
+    return Status::err(DG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPST(expr)                                          \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return hip_status(_e, #expr);  \
+    } while (0)
+
+Status ensure_device_ready() {
+    static std::mutex mu;
+    static bool done[64] = {false};
+    std::lock_guard<std::mutex> lk(mu);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return Status::err(DG_ERR_HIP, "no GPU device visible");
+    int dev = 0;
+    HIPST(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return Status::err(DG_ERR_HIP, "device id out of range");
+    if (!done[dev]) {
+        HIPST(upload_tables());
+        done[dev] = true;
+    }
+    return Status::ok();
+}
+
+PhysP make_phys(const dg_problem_desc& d) {
+    PhysP ph{};
+    ph.gamma = d.gamma;
+    ph.ax = d.ax;
+    ph.ay = d.ay;
+    ph.quirks = d.quirks;
+    return ph;
+}
+
+IcP make_ic(const dg_problem_desc& d, double Lx, double Ly) {
+    IcP ic{};
+    ic.kind = d.ic;
+    ic.nv = d.n_vortex;
+    ic.gamma = d.gamma;
+    ic.gx = d.gauss_x;
+    ic.gy = d.gauss_y;
+    ic.gk = d.gauss_k;
+    ic.Lx = Lx;
+    ic.Ly = Ly;
+    ic.rho_inf = d.rho_inf;
+    ic.u_inf = d.u_inf;
+    ic.v_inf = d.v_inf;
+    ic.p_inf = d.p_inf;
+    for (int k = 0; k < 4; k++) {
+        ic.beta[k] = d.vortex_beta[k];
+        ic.vx[k] = d.vortex_x[k];
+        ic.vy[k] = d.vortex_y[k];
+        ic.constant[k] = d.constant[k];
+    }
+    return ic;
+}
+
+Status check_problem(const dg_problem_desc& d, int p, int precision) {
+    if (p < 0 || p > 5) return Status::err(DG_ERR_ARG, "order p must be in 0..5");
+    if (precision != DG_FP64 && precision != DG_FP32) return Status::err(DG_ERR_ARG, "precision must be DG_FP64 or DG_FP32");
+    if (d.physics != DG_PHYS_ADVECTION && d.physics != DG_PHYS_EULER)
+        return Status::err(DG_ERR_ARG, "physics must be DG_PHYS_ADVECTION or DG_PHYS_EULER");
+    if (d.ic < 0 || d.ic > 2) return Status::err(DG_ERR_ARG, "unknown initial condition");
+    if (d.physics == DG_PHYS_ADVECTION && d.ic == DG_IC_VORTEX)
+        return Status::err(DG_ERR_ARG, "vortex initial condition needs Euler physics");
+    if (d.physics == DG_PHYS_EULER && d.ic == DG_IC_GAUSSIAN)
+        return Status::err(DG_ERR_ARG, "Gaussian pulse initial condition is for advection");
+    if (d.ic == DG_IC_VORTEX && (d.n_vortex < 1 || d.n_vortex > DG_MAX_VORTEX))
+        return Status::err(DG_ERR_ARG, "n_vortex must be in 1..4");
+    if (d.physics == DG_PHYS_EULER && !(d.gamma > 1.0)) return Status::err(DG_ERR_ARG, "gamma must exceed 1");
+    if (d.quirks & ~3) return Status::err(DG_ERR_ARG, "unknown quirk bits");
+    return Status::ok();
+}
+
+template <typename R>
+class SolverT : public Solver {
+   public:
+    std::vector<int> perm, inv;   // perm[internal] = caller's element id
+    PhysP phys{};
+    IcP ic{};
+    hipStream_t stream = nullptr;
+    R *c = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr;
+    R *geo_e = nullptr, *geo_f = nullptr;
+    int* nbr = nullptr;
+    double *V = nullptr, *Jd = nullptr, *lam_part = nullptr, *audit_part = nullptr;
+    Ctrl* ctrl = nullptr;
+    Ctrl hctrl{};
+    LaunchShape shape;
+    int kernel_req = 0;
+    bool lam_valid = false;
+    hipGraphExec_t gexec = nullptr;
+    int graph_scheme = 0;
+    int64_t bytes = 0;
+    int audit_blocks = 0;
+    int lam_capacity = 0;
+
+    ~SolverT() override {
+        if (gexec) hipGraphExecDestroy(gexec);
+        void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, lam_part, audit_part, ctrl};
+        for (void* b : bufs)
+            if (b) hipFree(b);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    size_t state_elems() const { return (size_t)N * NP * Kp; }
+
+    Status alloc(void** p, size_t n) {
+        HIPST(hipMalloc(p, n));
+        bytes += (int64_t)n;
+        return Status::ok();
+    }
+
+    Status init(const Mesh& m, const dg_problem_desc& pr, int p_) {
+        prob = pr;
+        p = p_;
+        N = pr.physics;
+        NP = n_modes(p);
+        NQ = n_quad_vol(p);
+        NQ1 = n_quad_face(p);
+        K = m.K;
+        S = m.S;
+        Kp = (K + 63) / 64 * 64;
+        precision = sizeof(R) == 8 ? DG_FP64 : DG_FP32;
+        phys = make_phys(pr);
+        ic = make_ic(pr, m.period_x, m.period_y);
+        if (m.ordered) {
+            perm.resize(K);
+            for (int i = 0; i < K; i++) perm[i] = i;
+        } else {
+            perm = morton_order(m);
+        }
+        inv.resize(K);
+        for (int i = 0; i < K; i++) inv[perm[i]] = i;
+        ElemGeometry g;
+        std::string why = elem_geometry(m, perm, pr.quirks, g);
+        if (!why.empty()) return Status::err(DG_ERR_UNSUPPORTED, why);
+        min_r = g.min_r;
+        Status st = ensure_device_ready();
+        if (!st) return st;
+        HIPST(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        const size_t M = state_elems();
+        if (!(st = alloc((void**)&c, M * sizeof(R)))) return st;
+        if (!(st = alloc((void**)&acc, M * sizeof(R)))) return st;
+        if (!(st = alloc((void**)&ua, M * sizeof(R)))) return st;
+        if (!(st = alloc((void**)&ub, M * sizeof(R)))) return st;
+        if (!(st = alloc((void**)&geo_e, 5 * (size_t)Kp * sizeof(R)))) return st;
+        if (!(st = alloc((void**)&geo_f, 9 * (size_t)Kp * sizeof(R)))) return st;
+        if (!(st = alloc((void**)&nbr, 3 * (size_t)Kp * sizeof(int)))) return st;
+        if (!(st = alloc((void**)&V, 6 * (size_t)Kp * sizeof(double)))) return st;
+        if (!(st = alloc((void**)&Jd, (size_t)Kp * sizeof(double)))) return st;
+        lam_capacity = Kp / 64 + 16;
+        if (!(st = alloc((void**)&lam_part, (size_t)lam_capacity * sizeof(double)))) return st;
+        audit_blocks = (K + 255) / 256;
+        if (!(st = alloc((void**)&audit_part, (size_t)audit_blocks * 8 * sizeof(double)))) return st;
+        if (!(st = alloc((void**)&ctrl, sizeof(Ctrl)))) return st;
+        HIPST(hipMemsetAsync(c, 0, M * sizeof(R), stream));
+        HIPST(hipMemsetAsync(acc, 0, M * sizeof(R), stream));
+        HIPST(hipMemsetAsync(ua, 0, M * sizeof(R), stream));
+        HIPST(hipMemsetAsync(ub, 0, M * sizeof(R), stream));
+        // geometry, padded entries replicate the last element
+        std::vector<R> he(5 * (size_t)Kp), hf(9 * (size_t)Kp);
+        std::vector<int> hn(3 * (size_t)Kp);
+        std::vector<double> hv(6 * (size_t)Kp), hj(Kp);
+        for (int i = 0; i < Kp; i++) {
+            const int e = std::min(i, K - 1);
+            he[i] = (R)g.xr[e];
+            he[Kp + i] = (R)g.yr[e];
+            he[2 * (size_t)Kp + i] = (R)g.xs[e];
+            he[3 * (size_t)Kp + i] = (R)g.ys[e];
+            he[4 * (size_t)Kp + i] = (R)g.J[e];
+            hj[i] = g.J[e];
+            for (int f = 0; f < 3; f++) {
+                hf[(3 * f + 0) * (size_t)Kp + i] = (R)g.fnx[f][e];
+                hf[(3 * f + 1) * (size_t)Kp + i] = (R)g.fny[f][e];
+                hf[(3 * f + 2) * (size_t)Kp + i] = (R)g.flen[f][e];
+                hn[f * (size_t)Kp + i] = g.nbr[f][e];
+            }
+            for (int v = 0; v < 6; v++) hv[v * (size_t)Kp + i] = g.V[v][e];
+        }
+        HIPST(hipMemcpyAsync(geo_e, he.data(), he.size() * sizeof(R), hipMemcpyHostToDevice, stream));
+        HIPST(hipMemcpyAsync(geo_f, hf.data(), hf.size() * sizeof(R), hipMemcpyHostToDevice, stream));
+        HIPST(hipMemcpyAsync(nbr, hn.data(), hn.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        HIPST(hipMemcpyAsync(V, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIPST(hipMemcpyAsync(Jd, hj.data(), hj.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        hctrl = Ctrl{};
+        hctrl.min_r = min_r;
+        hctrl.two_p1 = 2. * p + 1.;
+        if (!(st = choose_shape(0))) return st;
+        if (!(st = push_ctrl())) return st;
+        HIPST(hipStreamSynchronize(stream));
+        return Status::ok();
+    }
+
+    Status choose_shape(int req) {
+        kernel_req = req;
+        int kind = req;
+        if (kind == 0) kind = (sizeof(R) == 4 && mfma_supported(N, p)) ? 2 : 1;
+        if (kind == 2 && !(sizeof(R) == 4 && mfma_supported(N, p)))
+            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs FP32 and p = 3 or 4");
+        if (kind != 1 && kind != 2) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1 or 2");
+        LaunchShape s;
+        s.kind = kind;
+        s.tile_elems = (kind == 2) ? MFMA_TILE_ELEMS : VALU_TILE_ELEMS;
+        s.ntiles = (K + s.tile_elems - 1) / s.tile_elems;
+        s.xcd_remap = 1;
+        s.grid = (s.ntiles + 7) / 8 * 8;
+        if (s.ntiles > lam_capacity) return Status::err(DG_ERR_ARG, "internal: wave-speed partial buffer too small");
+        shape = s;
+        hctrl.nparts = s.ntiles;
+        lam_valid = false;
+        if (gexec) {
+            hipGraphExecDestroy(gexec);
+            gexec = nullptr;
+        }
+        return Status::ok();
+    }
+
+    Status push_ctrl() {
+        HIPST(hipMemcpyAsync(ctrl, &hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
+        return Status::ok();
+    }
+
+    Status pull_ctrl() {
+        HIPST(hipMemcpyAsync(&hctrl, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+        HIPST(hipStreamSynchronize(stream));
+        return Status::ok();
+    }
+
+    Status stage(int kind, const R* u, const R* cc, R* out) {
+        StageArgs<R> a{};
+        a.u = u;
+        a.c = cc;
+        a.acc = acc;
+        a.out = out;
+        a.geo_e = geo_e;
+        a.geo_f = geo_f;
+        a.nbr = nbr;
+        a.ctrl = ctrl;
+        a.lam_part = lam_part;
+        a.K = K;
+        a.Kp = Kp;
+        a.kind = kind;
+        a.ntiles = shape.ntiles;
+        a.xcd_remap = shape.xcd_remap;
+        a.phys = phys;
+        hipError_t e;
+        if constexpr (sizeof(R) == 4) {
+            if (shape.kind == 2)
+                e = launch_stage_mfma(N, p, a, dim3(shape.grid), stream);
+            else
+                e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), stream);
+        } else {
+            e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), stream);
+        }
+        return hip_status(e, "stage kernel launch");
+    }
+
+    Status ensure_lambda() {
+        if (lam_valid) return Status::ok();
+        const double b0 = build_tables(p).basis[0];
+        HIPST(launch_lambda<R>(N, c, K, Kp, NP, shape.tile_elems, shape.ntiles, shape.xcd_remap, b0, phys, lam_part,
+                               dim3(shape.grid), stream));
+        lam_valid = true;
+        return Status::ok();
+    }
+
+    // the stages of one step; the step's dt is already in ctrl
+    Status enqueue_stages(int scheme) {
+        Status st;
+        if (scheme == DG_RK2) {
+            if (!(st = stage(ST_RK2_1, c, c, ua))) return st;
+            return stage(ST_RK2_2, ua, c, c);
+        }
+        if (!(st = stage(ST_RK4_1, c, c, ua))) return st;
+        if (!(st = stage(ST_RK4_2, ua, c, ub))) return st;
+        if (!(st = stage(ST_RK4_3, ub, c, ua))) return st;
+        return stage(ST_RK4_4, ua, c, c);
+    }
+
+    Status enqueue_cfl_step(int scheme, bool use_graph) {
+        if (use_graph) {
+            if (!gexec || graph_scheme != scheme) {
+                if (gexec) {
+                    hipGraphExecDestroy(gexec);
+                    gexec = nullptr;
+                }
+                hipGraph_t g = nullptr;
+                HIPST(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                Status st = hip_status(launch_dt(ctrl, lam_part, stream), "dt kernel");
+                if (st) st = enqueue_stages(scheme);
+                hipError_t ee = hipStreamEndCapture(stream, &g);
+                if (!st) {
+                    if (g) hipGraphDestroy(g);
+                    return st;
+                }
+                HIPST(ee);
+                hipError_t ei = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                HIPST(ei);
+                graph_scheme = scheme;
+            }
+            HIPST(hipGraphLaunch(gexec, stream));
+            return Status::ok();
+        }
+        HIPST(launch_dt(ctrl, lam_part, stream));
+        return enqueue_stages(scheme);
+    }
+
+    Status begin_run(double t_end, bool has_end, double cfl) {
+        hctrl.t = t;
+        hctrl.steps = steps;
+        hctrl.t_end = t_end;
+        hctrl.has_end = has_end ? 1 : 0;
+        hctrl.done = 0;
+        hctrl.active = 0;
+        hctrl.cfl = cfl;
+        hctrl.min_r = min_r;
+        hctrl.two_p1 = 2. * p + 1.;
+        hctrl.nparts = shape.ntiles;
+        Status st = push_ctrl();
+        if (!st) return st;
+        return ensure_lambda();
+    }
+
+    Status project_ic() override {
+        HIPST(launch_project_ic<R>(N, p, c, V, K, Kp, ic, stream));
+        HIPST(hipStreamSynchronize(stream));
+        t = 0;
+        steps = 0;
+        lam_valid = false;
+        return Status::ok();
+    }
+
+    Status upload_ref(const double* c_ref, R* dst) {
+        std::vector<R> h(state_elems(), R(0));
+        for (int v = 0; v < N; v++)
+            for (int m = 0; m < NP; m++) {
+                const double* src = c_ref + ((size_t)v * NP + m) * K;
+                R* d = h.data() + ((size_t)v * NP + m) * Kp;
+                for (int i = 0; i < K; i++) d[i] = (R)src[perm[i]];
+                for (int i = K; i < Kp; i++) d[i] = d[K - 1];
+            }
+        HIPST(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice, stream));
+        HIPST(hipStreamSynchronize(stream));
+        return Status::ok();
+    }
+
+    Status download_ref(const R* src, double* c_ref) {
+        std::vector<R> h(state_elems());
+        HIPST(hipMemcpyAsync(h.data(), src, h.size() * sizeof(R), hipMemcpyDeviceToHost, stream));
+        HIPST(hipStreamSynchronize(stream));
+        for (int v = 0; v < N; v++)
+            for (int m = 0; m < NP; m++) {
+                double* d = c_ref + ((size_t)v * NP + m) * K;
+                const R* s = h.data() + ((size_t)v * NP + m) * Kp;
+                for (int i = 0; i < K; i++) d[perm[i]] = (double)s[i];
+            }
+        return Status::ok();
+    }
+
+    Status set_state(const double* c_ref) override {
+        Status st = upload_ref(c_ref, c);
+        t = 0;
+        steps = 0;
+        lam_valid = false;
+        return st;
+    }
+
+    Status get_state(double* c_ref) override { return download_ref(c, c_ref); }
+
+    Status rhs(const double* u_ref, double* k_ref, double dt) override {
+        Status st = upload_ref(u_ref, ub);
+        if (!st) return st;
+        HIPST(launch_set_dt(ctrl, dt, 0, stream));
+        if (!(st = stage(ST_RHS, ub, ub, ua))) return st;
+        return download_ref(ua, k_ref);
+    }
+
+    Status rk_step(double dt, int scheme) override {
+        if (scheme != DG_RK4 && scheme != DG_RK2) return Status::err(DG_ERR_ARG, "scheme must be DG_RK4 or DG_RK2");
+        HIPST(launch_set_dt(ctrl, dt, 1, stream));
+        Status st = enqueue_stages(scheme);
+        if (!st) return st;
+        lam_valid = true;  // the last stage left the new state's wave speeds
+        HIPST(hipStreamSynchronize(stream));
+        t += dt;
+        steps += 1;
+        return Status::ok();
+    }
+
+    Status cfl_dt(double cfl, double* dt) override {
+        Status st = ensure_lambda();
+        if (!st) return st;
+        std::vector<double> h(shape.ntiles);
+        HIPST(hipMemcpyAsync(h.data(), lam_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPST(hipStreamSynchronize(stream));
+        double mx = 0;
+        for (double x : h) mx = std::max(mx, x);
+        *dt = cfl * min_r / mx / (2. * p + 1.);
+        return Status::ok();
+    }
+
+    Status integrate(double t_end, double cfl, int scheme, int64_t* nsteps) override {
+        if (scheme != DG_RK4 && scheme != DG_RK2) return Status::err(DG_ERR_ARG, "scheme must be DG_RK4 or DG_RK2");
+        if (!(cfl > 0)) return Status::err(DG_ERR_ARG, "cfl must be positive");
+        const int64_t s0 = steps;
+        Status st = begin_run(t_end, true, cfl);
+        if (!st) return st;
+        int chunk = 8;
+        while (true) {
+            for (int i = 0; i < chunk; i++)
+                if (!(st = enqueue_cfl_step(scheme, true))) return st;
+            if (!(st = pull_ctrl())) return st;
+            if (hctrl.done || !(hctrl.t < t_end)) break;
+            chunk = std::min(chunk * 2, 64);
+        }
+        t = hctrl.t;
+        steps = hctrl.steps;
+        if (nsteps) *nsteps = steps - s0;
+        return Status::ok();
+    }
+
+    Status run_steps(int nsteps, double cfl, int scheme, bool use_graph) override {
+        if (scheme != DG_RK4 && scheme != DG_RK2) return Status::err(DG_ERR_ARG, "scheme must be DG_RK4 or DG_RK2");
+        if (!(cfl > 0)) return Status::err(DG_ERR_ARG, "cfl must be positive");
+        Status st = begin_run(0.0, false, cfl);
+        if (!st) return st;
+        for (int i = 0; i < nsteps; i++)
+            if (!(st = enqueue_cfl_step(scheme, use_graph))) return st;
+        if (!(st = pull_ctrl())) return st;
+        t = hctrl.t;
+        steps = hctrl.steps;
+        return Status::ok();
+    }
+
+    Status audit(dg_audit_out* out) override {
+        const double b0 = build_tables(p).basis[0];
+        HIPST(launch_audit<R>(N, c, Jd, K, Kp, NP, b0, phys.gamma, audit_part, audit_blocks, stream));
+        std::vector<double> h((size_t)audit_blocks * 8);
+        HIPST(hipMemcpyAsync(h.data(), audit_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPST(hipStreamSynchronize(stream));
+        dg_audit_out o{};
+        double rho = 1e300, pr = 1e300, bad = 0;
+        for (int b = 0; b < audit_blocks; b++) {
+            for (int v = 0; v < 4; v++) o.mass[v] += h[b * 8 + v];
+            rho = std::min(rho, h[b * 8 + 4]);
+            pr = std::min(pr, h[b * 8 + 5]);
+            bad += h[b * 8 + 6];
+        }
+        o.min_density = rho;
+        o.min_pressure = (N == 4) ? pr : INFINITY;
+        o.n_bad = (int64_t)bad;
+        o.t = t;
+        o.steps = steps;
+        *out = o;
+        return Status::ok();
+    }
+
+    Status eval_vertices(int var, double* uv) override {
+        if (var < 0 || var >= N) return Status::err(DG_ERR_ARG, "variable index out of range");
+        std::vector<double> cr(state_elems());
+        Status st = get_state(cr.data());
+        if (!st) return st;
+        Tables tb = build_tables(p);
+        for (int e = 0; e < K; e++) {
+            double a = 0, b = 0, d = 0;
+            for (int i = 0; i < NP; i++) {
+                const double ci = cr[((size_t)var * NP + i) * K + e];
+                a += ci * tb.vertex[i * 3 + 0];
+                b += ci * tb.vertex[i * 3 + 1];
+                d += ci * tb.vertex[i * 3 + 2];
+            }
+            uv[3 * (size_t)e] = a;
+            uv[3 * (size_t)e + 1] = b;
+            uv[3 * (size_t)e + 2] = d;
+        }
+        return Status::ok();
+    }
+
+    Status time_stage_kernels(int nsteps, double cfl, double* out) override {
+        Status st = begin_run(0.0, false, cfl);
+        if (!st) return st;
+        const int nst = 4 * nsteps;
+        std::vector<hipEvent_t> ev(2 * nst + 2 * nsteps + 2);
+        for (auto& e : ev) HIPST(hipEventCreate(&e));
+        int k = 0;
+        HIPST(hipEventRecord(ev[2 * nst + 2 * nsteps], stream));
+        for (int s = 0; s < nsteps; s++) {
+            HIPST(hipEventRecord(ev[2 * nst + 2 * s], stream));
+            HIPST(launch_dt(ctrl, lam_part, stream));
+            HIPST(hipEventRecord(ev[2 * nst + 2 * s + 1], stream));
+            const int kinds[4] = {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4};
+            const R* ins[4] = {c, ua, ub, ua};
+            R* outs[4] = {ua, ub, ua, c};
+            for (int j = 0; j < 4; j++) {
+                HIPST(hipEventRecord(ev[2 * k], stream));
+                if (!(st = stage(kinds[j], ins[j], c, outs[j]))) return st;
+                HIPST(hipEventRecord(ev[2 * k + 1], stream));
+                k++;
+            }
+        }
+        HIPST(hipEventRecord(ev[2 * nst + 2 * nsteps + 1], stream));
+        HIPST(hipStreamSynchronize(stream));
+        double sum = 0, sumdt = 0;
+        for (int i = 0; i < nst; i++) {
+            float ms = 0;
+            HIPST(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+            sum += ms;
+        }
+        for (int s = 0; s < nsteps; s++) {
+            float ms = 0;
+            HIPST(hipEventElapsedTime(&ms, ev[2 * nst + 2 * s], ev[2 * nst + 2 * s + 1]));
+            sumdt += ms;
+        }
+        float tot = 0;
+        HIPST(hipEventElapsedTime(&tot, ev[2 * nst + 2 * nsteps], ev[2 * nst + 2 * nsteps + 1]));
+        for (auto& e : ev) hipEventDestroy(e);
+        out[0] = sum / nst;
+        out[1] = sumdt / nsteps;
+        out[2] = tot;
+        out[3] = nst;
+        if (!(st = pull_ctrl())) return st;
+        t = hctrl.t;
+        steps = hctrl.steps;
+        return Status::ok();
+    }
+
+    Status set_kernel(int kind) override {
+        Status st = choose_shape(kind);
+        if (!st) return st;
+        return push_ctrl();
+    }
+    int kernel_kind() const override { return shape.kind; }
+    int64_t device_bytes() const override { return bytes; }
+    Status sync() override {
+        HIPST(hipStreamSynchronize(stream));
+        return Status::ok();
+    }
+};
+
+Status Solver::create(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out) {
+    Status st = check_problem(prob, p, precision);
+    if (!st) return st;
+    if (precision == DG_FP64) {
+        auto* s = new SolverT<double>();
+        st = s->init(m, prob, p);
+        if (!st) {
+            delete s;
+            return st;
+        }
+        *out = s;
+    } else {
+        auto* s = new SolverT<float>();
+        st = s->init(m, prob, p);
+        if (!st) {
+            delete s;
+            return st;
+        }
+        *out = s;
+    }
+    return Status::ok();
+}
+
+}  // namespace dgmi

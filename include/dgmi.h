@@ -1,0 +1,241 @@
+/*
+ * dgmi.h -- C ABI of the MI355X-native DG right-hand side and explicit RK step
+ * for 2D advection and Euler on unstructured triangle meshes.
+ *
+ * Every entry point returns a dg_status (0 = DG_OK).  On failure the reason is
+ * available from dg_last_error() (thread-local, valid until the next call on
+ * the same thread).  No entry point calls exit() or aborts.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   read_mesh            src/2d/conservation/conserv.cu:118   -> dg_mesh_read_pmsh / dg_mesh_from_arrays
+ *   init_gpu             src/2d/conservation/conserv.cu:194   -> dg_solver_create
+ *   set_quadrature       src/2d/conservation/conserv.cu:37    -> dg_solver_create (internal), dg_tables
+ *   preval_basis         src/2d/basis.cu:150                  -> dg_solver_create (internal), dg_tables
+ *   preval_* geometry    src/2d/conservation/conserv_kernels.cu:338-514, main.cu:84-149
+ *                                                             -> dg_solver_create (internal), dg_mesh_geometry
+ *   init_conditions      src/2d/conservation/conserv_kernels.cu:167 -> dg_project_ic
+ *   eval_surface/eval_volume/eval_rhs
+ *                        src/2d/conservation/conserv_kernels.cu:697,766; time_integrator.cu:130
+ *                                                             -> dg_rhs
+ *   rk_tempstorage/rk4/rk2 time_integrator.cu:194,212,445,458  -> dg_rk_step
+ *   eval_global_lambda + host max + dt
+ *                        src/2d/conservation/euler/euler.cu:252; time_integrator.cu:254-263
+ *                                                             -> dg_cfl_dt
+ *   time_integrate_rk4 / time_integrate_rk2
+ *                        src/2d/conservation/time_integrator.cu:221,466 -> dg_integrate
+ *   eval_u               src/2d/conservation/conserv_kernels.cu:818 -> dg_eval_vertices
+ *   free_gpu             src/2d/conservation/conserv.cu:323   -> dg_solver_destroy
+ *   run_dgcuda           src/2d/conservation/main.cu:8        -> the dgmi command-line driver
+ *
+ * State layout at the boundary ("reference layout"): c[var][mode][elem] in
+ * the caller's element numbering, index num_elem*n_p*var + mode*num_elem + elem
+ * (conserv_kernels.cu:183), always FP64 on the host side.
+ */
+#ifndef DGMI_H
+#define DGMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DG_OK = 0,
+    DG_ERR_ARG = 1,         /* invalid argument */
+    DG_ERR_IO = 2,          /* file could not be read or written */
+    DG_ERR_PARSE = 3,       /* malformed mesh file */
+    DG_ERR_HIP = 4,         /* HIP runtime error or no GPU */
+    DG_ERR_UNSUPPORTED = 5, /* valid but not supported (e.g. boundary sides) */
+    DG_ERR_NUMERIC = 6,     /* non-finite state or non-positive density / pressure */
+    DG_ERR_BUDGET = 7,      /* request exceeds the memory budget */
+    DG_ERR_STATE = 8        /* call out of order */
+} dg_status;
+
+/* physics: the value is the number of conserved variables (the reference's local_N) */
+enum { DG_PHYS_ADVECTION = 1, DG_PHYS_EULER = 4 };
+/* bulk arithmetic of the state and the kernels; geometry is always computed in FP64 */
+enum { DG_FP64 = 0, DG_FP32 = 1 };
+/* time integrators, numbered as the reference's RK4 / RK2 (conserv.cu:19-20) */
+enum { DG_RK4 = 1, DG_RK2 = 2 };
+/* initial conditions */
+enum { DG_IC_CONSTANT = 0, DG_IC_GAUSSIAN = 1, DG_IC_VORTEX = 2 };
+/* optional reproduction of the reference's float roundings:
+ *  FLOAT_SQRT: sound speed and cell wave speed through a float square root
+ *              (euler.cu:80, 269);
+ *  FLOAT_GEOM: side lengths, normals and inscribed diameters in float
+ *              (conserv_kernels.cu:346, 364-371, 434). */
+enum { DG_QUIRK_FLOAT_SQRT = 1, DG_QUIRK_FLOAT_GEOM = 2 };
+
+#define DG_MAX_VORTEX 4
+
+/* Problem description (replaces the compile-time plug-in of the reference's
+ * problem translation units, e.g. euler/supersonic/supersonic.cu:10-38). */
+typedef struct {
+    int physics;        /* DG_PHYS_ADVECTION or DG_PHYS_EULER */
+    int ic;             /* DG_IC_* */
+    int quirks;         /* DG_QUIRK_* bit set */
+    int n_vortex;       /* DG_IC_VORTEX: number of superposed vortices (1..DG_MAX_VORTEX) */
+    double gamma;       /* ratio of specific heats (Euler) */
+    double ax, ay;      /* advection velocity */
+    double gauss_x, gauss_y, gauss_k; /* DG_IC_GAUSSIAN: exp(-k |x - x_c|^2) */
+    double rho_inf, u_inf, v_inf, p_inf; /* DG_IC_VORTEX background state */
+    double vortex_beta[DG_MAX_VORTEX], vortex_x[DG_MAX_VORTEX], vortex_y[DG_MAX_VORTEX];
+    double constant[4]; /* DG_IC_CONSTANT: conserved state */
+} dg_problem_desc;
+
+typedef struct {
+    double mass[4];         /* integral of each conserved variable (FP64 sum) */
+    double min_density;     /* minimum cell-mean density (Euler) or value (advection) */
+    double min_pressure;    /* minimum cell-mean pressure (Euler; +inf for advection) */
+    int64_t n_bad;          /* cells with non-finite or non-positive density / pressure */
+    double t;               /* simulated time reached */
+    int64_t steps;          /* RK steps taken since the last dg_project_ic / dg_set_state */
+} dg_audit_out;
+
+typedef struct dg_mesh dg_mesh;
+typedef struct dg_solver dg_solver;
+typedef struct dg_batch dg_batch;
+typedef struct dg_server dg_server;
+
+const char* dg_last_error(void);
+/* library version string */
+const char* dg_version(void);
+/* number of visible GPUs (0 when none) */
+int dg_device_count(void);
+
+/* ---------------- meshes (host only) ---------------- */
+/* reads the conservation-framework .pmsh dialect (conserv.cu:118-192) */
+int dg_mesh_read_pmsh(const char* path, dg_mesh** out);
+/* builds a mesh from the arrays read_mesh returns, in its order:
+ * V[6K] = V1x|V1y|V2x|V2y|V3x|V3y, elem_s[3K] = elem_s1|elem_s2|elem_s3,
+ * sides[4S] = x1|y1|x2|y2, side_int[4S] = left_elem|right_elem|left_side_number|right_side_number.
+ * period_x/period_y: periodic extent (0 if not periodic). */
+int dg_mesh_from_arrays(int num_elem, int num_sides, const double* V, const int* elem_s, const double* sides,
+                        const int* side_int, double period_x, double period_y, dg_mesh** out);
+/* periodic n_x x n_y jittered-square mesh on [x0,x1]x[y0,y1], 2*n_x*n_y triangles
+ * (no counterpart in the reference, which reads meshes made by gmsh) */
+int dg_mesh_periodic_jittered(int nx, int ny, double x0, double x1, double y0, double y1, double jitter,
+                              uint64_t seed, dg_mesh** out);
+int dg_mesh_write_pmsh(const dg_mesh* m, const char* path);
+int dg_mesh_info(const dg_mesh* m, int* num_elem, int* num_sides, int* num_boundary_sides);
+/* the read_mesh arrays (any pointer may be NULL) */
+int dg_mesh_get_arrays(const dg_mesh* m, double* V, int* elem_s, double* sides, int* side_int);
+/* FP64 geometry as the reference precomputes it: geo_e[5K] = J|xr|yr|xs|ys,
+ * geo_s[3S] = len|Nx|Ny, *min_r = minimum inscribed-circle diameter */
+int dg_mesh_geometry(const dg_mesh* m, int quirks, double* geo_e, double* geo_s, double* min_r);
+void dg_mesh_destroy(dg_mesh* m);
+
+/* FP64 basis and quadrature tables for order p in the reference's preval_basis
+ * layouts (basis.cu:150-226): r1,r2,w[n_q]; s_r,w1[n_q1]; basis,gx,gy[n_p*n_q];
+ * side[3*n_p*n_q1]; vertex[3*n_p].  sizes[0..2] = n_p, n_q, n_q1. */
+int dg_tables(int p, int* sizes, double* r1, double* r2, double* w, double* s_r, double* w1, double* basis,
+              double* gx, double* gy, double* side, double* vertex);
+
+/* ---------------- one solve on the GPU ---------------- */
+/* p in 0..5 as the reference's -n option; precision DG_FP64 / DG_FP32 */
+int dg_solver_create(const dg_mesh* m, const dg_problem_desc* prob, int p, int precision, dg_solver** out);
+void dg_solver_destroy(dg_solver* s);
+/* sizes[0..5] = num_elem, num_sides, n_p, n_q, n_q1, N */
+int dg_solver_info(const dg_solver* s, int* sizes, double* min_r);
+/* L2 projection of the problem's initial condition (init_conditions); resets t and steps */
+int dg_project_ic(dg_solver* s);
+/* state I/O in the reference layout, FP64 host buffers of N*n_p*num_elem; set resets t and steps */
+int dg_set_state(dg_solver* s, const double* c_ref);
+int dg_get_state(dg_solver* s, double* c_ref);
+/* k = dt * M^-1 R(u) exactly as eval_surface + eval_volume + eval_rhs compute it
+ * (M = J*I); u and k are host buffers in the reference layout; t is unused on
+ * periodic meshes and kept for the reference's signature */
+int dg_rhs(dg_solver* s, const double* u_ref, double* k_ref, double t, double dt);
+/* one RK step of size dt on the resident state (scheme DG_RK4 / DG_RK2) */
+int dg_rk_step(dg_solver* s, double dt, int scheme);
+/* dt = cfl * min_r / max_K lambda_K / (2p+1) from the resident state */
+int dg_cfl_dt(dg_solver* s, double cfl, double* dt);
+/* time_integrate_rk4/rk2: from the current t to t_end with CFL steps, the last
+ * one clamped to t_end; dt is computed on the device every step */
+int dg_integrate(dg_solver* s, double t_end, double cfl, int scheme, int64_t* steps, double* t_out);
+/* nsteps CFL steps with no end time (benchmarking); use_graph replays a
+ * captured hipGraph of one step */
+int dg_run_steps(dg_solver* s, int nsteps, double cfl, int scheme, int use_graph);
+int dg_audit(dg_solver* s, dg_audit_out* out);
+/* uv[3*num_elem]: variable var at the three vertices of each element (eval_u) */
+int dg_eval_vertices(dg_solver* s, int var, double* uv);
+int dg_solver_sync(dg_solver* s);
+/* kernel selection for the stage kernels: 0 = automatic (tuned table),
+ * 1 = one element per lane (VALU), 2 = matrix-core tiles (FP32, p = 3, 4) */
+int dg_solver_set_kernel(dg_solver* s, int kind);
+int dg_solver_get_kernel(const dg_solver* s, int* kind);
+/* Times nsteps RK4 steps with HIP events on the solver's stream, one event
+ * pair around every stage-kernel launch.  out[0] = mean stage-kernel ms,
+ * out[1] = mean dt-kernel ms, out[2] = total ms, out[3] = number of stage
+ * launches timed. */
+int dg_time_stage_kernels(dg_solver* s, int nsteps, double cfl, double* out);
+/* device buffer bytes held by the solver */
+int dg_solver_device_bytes(const dg_solver* s, int64_t* bytes);
+
+/* ---------------- many solves, one persistent launch per RK stage ---------------- */
+int dg_batch_create(int precision, dg_batch** out);
+void dg_batch_destroy(dg_batch* b);
+/* adds one solve (its own mesh, problem, order p, end time); returns its slot id */
+int dg_batch_add(dg_batch* b, const dg_mesh* m, const dg_problem_desc* prob, int p, double t_end, int* slot);
+/* projects the initial conditions of every solve and resets their clocks */
+int dg_batch_project_ic(dg_batch* b);
+/* nsteps RK4 steps on every unfinished solve (each with its own CFL dt),
+ * one persistent launch per stage, replayed from a hipGraph when use_graph */
+int dg_batch_run_steps(dg_batch* b, int nsteps, double cfl, int use_graph);
+/* runs every solve to its end time; *steps = number of batch steps issued */
+int dg_batch_integrate(dg_batch* b, double cfl, int use_graph, int64_t* steps);
+int dg_batch_get_state(dg_batch* b, int slot, double* c_ref);
+int dg_batch_audit(dg_batch* b, int slot, dg_audit_out* out);
+int dg_batch_info(const dg_batch* b, int* n_solves, int64_t* total_elems, int64_t* device_bytes);
+
+/* ---------------- serving: continuous batching ---------------- */
+typedef struct {
+    int precision;          /* DG_FP32 or DG_FP64 */
+    int max_slots;          /* solves resident at once */
+    int64_t hbm_budget;     /* bytes of device memory the resident solves may hold */
+    int steps_per_round;    /* batch steps between admissions / completions */
+    double cfl;
+    int use_graph;
+} dg_server_config;
+
+typedef struct {
+    int nx, ny;             /* jittered periodic mesh of 2*nx*ny triangles */
+    double x0, x1, y0, y1, jitter;
+    uint64_t mesh_seed;
+    int p;
+    double t_end;
+    dg_problem_desc prob;
+    double arrival_s;       /* arrival time, seconds on the server's clock (0 = now) */
+} dg_request;
+
+typedef struct {
+    uint64_t id;
+    int status;             /* dg_status of this request */
+    int64_t steps;
+    double t;
+    double arrival_s, admit_s, finish_s; /* server clock, seconds */
+    double mass[4];         /* FP64 audit of the final state */
+    double mass0[4];        /* FP64 audit of the initial state */
+    int64_t n_bad;
+} dg_result;
+
+int dg_server_create(const dg_server_config* cfg, dg_server** out);
+/* queues a request; it is admitted when a slot and budget are free */
+int dg_server_submit(dg_server* sv, const dg_request* req, uint64_t* id);
+/* copies up to max finished results (oldest first) and removes them */
+int dg_server_poll(dg_server* sv, dg_result* out, int max, int* n);
+/* final state of a finished request, reference layout (requires keep_states) */
+int dg_server_get_state(dg_server* sv, uint64_t id, double* c_ref, int64_t n);
+int dg_server_keep_states(dg_server* sv, int keep);
+/* blocks until every submitted request has finished */
+int dg_server_drain(dg_server* sv);
+/* seconds on the server's clock */
+double dg_server_now(dg_server* sv);
+void dg_server_destroy(dg_server* sv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

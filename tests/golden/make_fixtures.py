@@ -1,0 +1,85 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures of tests/golden from the reference's own files.
+
+Runs only where the reference checkout exists (default /root/reference); the
+fixtures it writes are committed, so the tests never read the reference.
+
+  ref_tables.json   basis_coeff rows/cols 0..20 (src/2d/basis.cu:2-47) and the
+                    2D / 1D quadrature rules (src/2d/quadrature.cu:7-169), the
+                    1D sqrt() expressions evaluated in IEEE double
+  t0/               the reference's projection golden vectors
+                    src/2d/tests/t0/alpha{a}_{mesh}{p}.out for the meshes
+                    canonical, box and supersimple, plus those meshes
+                    (src/2d/mesh/*.pmsh, advection-prototype 6-field dialect)
+  sv1.pmsh          conservation-dialect mesh with boundary sides
+                    (src/2d/conservation/euler/supersonic/mesh/sv1.pmsh)
+"""
+import json
+import math
+import os
+import re
+import shutil
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def c_values(body):
+    vals, depth, cur = [], 0, ""
+    for ch in body:
+        if ch == "(":
+            depth += 1
+        elif ch == ")":
+            depth -= 1
+        if ch == "," and depth == 0:
+            vals.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    vals.append(cur)
+    out = []
+    for v in vals:
+        v = v.strip()
+        if v:
+            out.append(float(eval(re.sub(r"\bsqrt\b", "math.sqrt", v), {"__builtins__": {}}, {"math": math})))
+    return out
+
+
+def main():
+    src2d = os.path.join(REF, "src", "2d")
+    basis = open(os.path.join(src2d, "basis.cu")).read()
+    body = re.search(r"basis_coeff\[size\]\[size\]\s*=\s*\{(.*?)\n\};", basis, re.S).group(1)
+    rows = [[float(x) for x in r.split(",")] for r in re.findall(r"\{([^{}]*)\}", body)]
+    quad = open(os.path.join(src2d, "quadrature.cu")).read()
+    q2 = {}
+    for d in range(1, 11):
+        b = re.search(r"double\s+quad_2d_degree%d\[\]\s*=\s*\{(.*?)\};" % d, quad, re.S).group(1)
+        q2[d] = c_values(b)
+    q1 = {}
+    for n in range(1, 9):
+        b = re.search(r"double\s+quad_1d_degree%d\[\]\s*=\s*\{(.*?)\};" % n, quad, re.S).group(1)
+        q1[n] = c_values(b)
+    data = {
+        "source": "src/2d/basis.cu:2-47, src/2d/quadrature.cu:7-169",
+        "basis_coeff": [r[:21] for r in rows[:21]],
+        "quad2d": {str(d): q2[d] for d in q2},
+        "quad1d": {str(n): q1[n] for n in q1},
+    }
+    with open(os.path.join(OUT, "ref_tables.json"), "w") as f:
+        json.dump(data, f, indent=0)
+    t0 = os.path.join(OUT, "t0")
+    os.makedirs(t0, exist_ok=True)
+    for mesh in ("canonical", "box", "supersimple"):
+        shutil.copy(os.path.join(src2d, "mesh", mesh + ".pmsh"), os.path.join(t0, mesh + ".pmsh"))
+        for a in range(4):
+            for p in range(a, 6):
+                name = "alpha%d_%s%d.out" % (a, mesh, p)
+                shutil.copy(os.path.join(src2d, "tests", "t0", name), os.path.join(t0, name))
+    shutil.copy(os.path.join(src2d, "conservation", "euler", "supersonic", "mesh", "sv1.pmsh"),
+                os.path.join(OUT, "sv1.pmsh"))
+    print("fixtures written to", OUT)
+
+
+if __name__ == "__main__":
+    main()
