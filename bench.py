@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Euler p=4 DOF.stage updates/s on one MI355X (the 3-vortex, 524,288-triangle
+periodic workload), with the stage kernel's HBM roofline and the FP64 CPU
+oracle's throughput beside it.
+
+  python bench.py [--gpus 1] [--steps 200] [--warmup 10]
+
+Prints one JSON line.  A step is one classical RK4 step (4 stages) of the
+whole mesh with the CFL time step computed on the device; the timed region
+replays the captured step graph `--steps` times with the state resident in
+HBM.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def vortices(seed, n=3, L=10.0):
+    rng = np.random.default_rng(seed)
+    return tuple((float(rng.uniform(2, 5)), float(rng.uniform(0, L)), float(rng.uniform(0, L))) for _ in range(n))
+
+
+def algorithmic_bytes_per_elem(N, n_p, s):
+    """Bytes one RK4 stage must move per element, averaged over the 4 stages:
+    state and RK registers 4.25*N*n_p*s (stage 1 reads c, writes acc and the
+    next input; stages 2-3 read input, c, acc and write acc and the next input;
+    stage 4 reads input, c, acc and writes c) plus the element's geometry
+    (xr, yr, xs, ys, J; per face nx, ny, len; 3 packed neighbour ints)."""
+    return 4.25 * N * n_p * s + 14 * s + 3 * 4
+
+
+def cpu_baseline(p, seed, n_patch=64, steps=40):
+    """FP64 serial oracle on a 64x64-cell patch of the same workload (same
+    domain size per cell, same vortex superposition), 1 thread."""
+    import dgcuda_amd as dg
+    import oracle as orc
+
+    h = 10.0 / 512
+    L = n_patch * h
+    vs = tuple((b, x * L / 10.0, y * L / 10.0) for b, x, y in vortices(seed))
+    m = dg.Mesh.periodic_jittered(n_patch, n_patch, 0.0, L, 0.0, L, 0.1, seed)
+    prob = dg.euler_vortex(vs)
+    o = orc.Oracle.from_mesh(m, p, prob)
+    u = o.project_ic(orc.ic_array(prob, m.period))
+    t0 = time.perf_counter()
+    u, _ = o.run_steps(u, steps, 0.3)
+    el = time.perf_counter() - t0
+    dof = m.num_elem * o.n_p * o.N
+    return dict(value=dof * 4 * steps / el, unit="DOF*stage/s", cores=1, kind="port",
+                sample="FP64 serial oracle, %dx%d-cell patch (%d triangles) of the workload, p=%d, %d RK4 steps, "
+                       "%.1f s" % (n_patch, n_patch, m.num_elem, p, steps, el))
+
+
+def load_traffic(p):
+    path = os.path.join(ROOT, "profiles", "traffic_euler_p%d_524k.json" % p)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512, help="cells per side (2*n*n triangles)")
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 VALU, 2 matrix cores")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="only run this many steps (for rocprof) and print nothing else")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import dgcuda_amd as dg
+
+    prec = dg.DG_FP32 if args.precision == "fp32" else dg.DG_FP64
+    s_bytes = 4 if prec == dg.DG_FP32 else 8
+    t_setup = time.perf_counter()
+    mesh = dg.Mesh.periodic_jittered(args.n, args.n, 0.0, 10.0, 0.0, 10.0, 0.1, args.seed)
+    vs = vortices(args.seed)
+    prob = dg.euler_vortex(vs)
+    s = dg.Solver(mesh, prob, args.p, prec)
+    if args.kernel:
+        s.set_kernel(args.kernel)
+    s.project_ic()
+    s.sync()
+    setup_s = time.perf_counter() - t_setup
+    if args.profile_steps:
+        s.run_steps(args.profile_steps, 0.3, use_graph=False)
+        s.sync()
+        print("profiled %d steps" % args.profile_steps, flush=True)
+        return
+
+    K, n_p, N = s.num_elem, s.n_p, s.N
+    dof = K * n_p * N
+    s.run_steps(args.warmup, 0.3, use_graph=True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run_steps(args.steps, 0.3, use_graph=True)
+    s.sync()
+    el = time.perf_counter() - t0
+    value = dof * 4 * args.steps / el
+    t_reached = s.audit()["t"]
+    state32 = None if args.no_accuracy else s.get_state()
+
+    # dominant kernel: the fused stage kernel, timed per launch with HIP
+    # events on the solver's stream
+    tk = s.time_stage_kernels(20, 0.3)
+    bpe = algorithmic_bytes_per_elem(N, n_p, s_bytes)
+    bytes_launch = bpe * K
+    achieved = bytes_launch / (tk["stage_ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(achieved / PEAK_HBM_GBS, 4), traffic=load_traffic(args.p),
+                kernel="stage (fused volume + LLF faces + M^-1 + RK update)",
+                stage_ms=round(tk["stage_ms"], 5), dt_kernel_ms=round(tk["dt_ms"], 5),
+                algorithmic_bytes_per_launch=int(bytes_launch), bytes_per_dof_stage=round(bpe / (n_p * N), 3))
+
+    extra = {}
+    if state32 is not None and prec == dg.DG_FP32:
+        s64 = dg.Solver(mesh, prob, args.p, dg.DG_FP64)
+        s64.project_ic()
+        n64, _ = s64.integrate(t_reached, 0.3)
+        import oracle as orc
+
+        err = orc.rel_l2(state32, s64.get_state())
+        extra["rel_l2_fp32_vs_fp64"] = float("%.3e" % err)
+        extra["accuracy_reference"] = ("GPU FP64 run to the same time (%d steps); the FP64 GPU path matches the "
+                                       "FP64 oracle to <=1e-12 in tests" % n64)
+        s64.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.p, args.seed)
+    bw_ms, bw = dg.bandwidth_probe(1 << 30, 10)
+    out = {
+        "metric": "Euler p=4 DOF*stage updates/s",
+        "value": value,
+        "unit": "DOF*stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if prec == dg.DG_FP32 else "fp64",
+        "data": "synthetic: seeded 3-vortex isentropic superposition on a seeded jittered periodic mesh",
+        "config": {"workload": "euler_rk4_p%d_3vortex_%dx%d_periodic" % (args.p, args.n, args.n),
+                   "triangles": K, "p": args.p, "n_p": n_p, "dof": dof, "cfl": 0.3, "integrator": "rk4",
+                   "kernel": s.kernel(), "geometry": "fp64-computed"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "copy_kernel_GBs": round(bw, 1),
+        "frac_of_copy": round(achieved / bw, 4),
+        "setup_s": round(setup_s, 3),
+        "t_reached": t_reached,
+        "host": platform.node(),
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

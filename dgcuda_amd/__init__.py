@@ -103,6 +103,7 @@ def lib():
             "dg_audit": (I, [VP, P(AuditOut)]), "dg_eval_vertices": (I, [VP, I, dp]),
             "dg_solver_sync": (I, [VP]), "dg_solver_set_kernel": (I, [VP, I]), "dg_solver_get_kernel": (I, [VP, ip]),
             "dg_time_stage_kernels": (I, [VP, I, D, dp]), "dg_solver_device_bytes": (I, [VP, P(I64)]),
+            "dg_bandwidth_probe": (I, [I64, I, dp]), "dg_selftest_mfma": (I, [dp]),
             "dg_batch_create": (I, [I, P(VP)]), "dg_batch_destroy": (None, [VP]),
             "dg_batch_add": (I, [VP, VP, P(ProblemDesc), I, D, ip]), "dg_batch_project_ic": (I, [VP]),
             "dg_batch_run_steps": (I, [VP, I, D, I]), "dg_batch_integrate": (I, [VP, D, I, P(I64)]),
@@ -141,6 +142,13 @@ def _ip(a):
 
 def device_count():
     return lib().dg_device_count()
+
+
+def bandwidth_probe(nbytes=1 << 30, iters=20):
+    """Streaming device copy: (ms per copy, GB/s counting read + write)."""
+    out = np.zeros(2)
+    _check(lib().dg_bandwidth_probe(nbytes, iters, _dp(out)))
+    return out[0], out[1]
 
 
 def n_modes(p):
@@ -440,3 +448,10 @@ def request(nx, p, t_end, prob, domain=(0.0, 10.0), jitter=0.1, mesh_seed=0, arr
     r.y0, r.y1 = domain
     r.jitter, r.mesh_seed, r.p, r.t_end, r.prob, r.arrival_s = jitter, mesh_seed, p, t_end, prob, arrival_s
     return r
+
+
+def selftest_mfma():
+    """Max abs error of a 16x16x4 FP32 matrix-core product vs the host."""
+    e = C.c_double()
+    _check(lib().dg_selftest_mfma(C.byref(e)))
+    return e.value

@@ -324,6 +324,62 @@ int dg_solver_device_bytes(const dg_solver* s, int64_t* bytes) {
     return DG_OK;
 }
 
+int dg_bandwidth_probe(int64_t bytes, int iters, double* out) {
+    GUARD_BEGIN
+    if (bytes <= 0 || bytes % 16 || iters <= 0 || !out) return fail(DG_ERR_ARG, "invalid argument");
+    Status st = ensure_device_ready();
+    if (!st) return ret(st);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (a) hipFree(a);
+        if (b) hipFree(b);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        if (s) hipStreamDestroy(s);
+    };
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, s);
+    if (e == hipSuccess) e = launch_stream_copy(a, b, bytes, s);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int i = 0; i < iters && e == hipSuccess; i++) e = launch_stream_copy(a, b, bytes, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    cleanup();
+    if (e != hipSuccess) return ret(hip_status(e, "bandwidth probe"));
+    out[0] = ms / iters;
+    out[1] = 2.0 * (double)bytes / (out[0] * 1e-3) / 1e9;
+    return DG_OK;
+    GUARD_END
+}
+
+int dg_selftest_mfma(double* max_err) {
+    GUARD_BEGIN
+    if (!max_err) return fail(DG_ERR_ARG, "null output");
+    Status st = ensure_device_ready();
+    if (!st) return ret(st);
+    float d[256];
+    hipError_t e = mfma_layout_probe(d);
+    if (e != hipSuccess) return ret(hip_status(e, "mfma probe"));
+    double err = 0;
+    for (int i = 0; i < 16; i++)
+        for (int j = 0; j < 16; j++) {
+            double want = 0;
+            for (int k = 0; k < 4; k++) want += ((i + 1) + 0.5 * k) * ((2 * k + 1) - 0.25 * j);
+            err = std::max(err, std::fabs(want - (double)d[i * 16 + j]));
+        }
+    *max_err = err;
+    return DG_OK;
+    GUARD_END
+}
+
 /* ---------------- batch ---------------- */
 int dg_batch_create(int precision, dg_batch** out) {
     GUARD_BEGIN

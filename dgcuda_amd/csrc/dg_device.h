@@ -38,8 +38,17 @@ struct TabAll {
     TabP<R, 5> p5;
 };
 
-extern __constant__ TabAll<double> c_tab64;
-extern __constant__ TabAll<float> c_tab32;
+// Each kernel translation unit holds its own copy (code objects are not
+// linked across units) and uploads it through its DGMI_TABLE_UPLOADER.
+static __constant__ TabAll<double> c_tab64;
+static __constant__ TabAll<float> c_tab32;
+
+#define DGMI_TABLE_UPLOADER(fname)                                                        \
+    hipError_t fname(const TabAll<double>& h64, const TabAll<float>& h32) {               \
+        hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab64), &h64, sizeof(h64));          \
+        if (e != hipSuccess) return e;                                                    \
+        return hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &h32, sizeof(h32));                 \
+    }
 
 template <typename R, int P>
 __device__ __forceinline__ const TabP<R, P>& tab();
@@ -73,12 +82,19 @@ struct Ctrl {
     double min_r;    // minimum inscribed-circle diameter of the mesh
     double two_p1;   // 2p + 1
     int64_t steps;   // steps taken
+    // maximum cell wave speed of the current state, as the bit pattern of a
+    // non-negative double (ordered like the value), accumulated with one
+    // atomic max per workgroup; reset by the dt kernel after it is read
+    unsigned long long lam_bits;
     int active;      // the stages of the current step run
     int done;        // t reached t_end
     int has_end;     // clamp to t_end
-    int nparts;      // number of wave-speed partials to reduce
-    int pad[2];
+    int pad;
 };
+
+__device__ __forceinline__ void lam_commit(double mx, unsigned long long* bits) {
+    if (mx > 0.0) atomicMax(bits, (unsigned long long)__double_as_longlong(mx));
+}
 
 struct PhysP {
     double gamma;
@@ -113,7 +129,7 @@ struct StageArgs {
     const R* geo_f;   // [9][Kp]: (nx, ny, len) for faces 0..2
     const int* nbr;   // [3][Kp]
     const Ctrl* ctrl;
-    double* lam_part; // per-tile maximum wave speed (last stage)
+    unsigned long long* lam_bits; // &ctrl->lam_bits (last stage)
     int K, Kp;
     int kind;
     int ntiles;       // tiles (workgroups) that carry elements

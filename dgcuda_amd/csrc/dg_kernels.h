@@ -17,22 +17,43 @@ struct IcP {
 };
 
 hipError_t upload_tables();
+hipError_t upload_tables_kernels(const TabAll<double>& h64, const TabAll<float>& h32);
+hipError_t upload_tables_mfma(const TabAll<double>& h64, const TabAll<float>& h32);
+hipError_t upload_tables_batch(const TabAll<double>& h64, const TabAll<float>& h32);
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
 
 template <typename R>
 hipError_t launch_stage_valu(int N, int p, const StageArgs<R>& a, dim3 grid, hipStream_t st);
 
-// matrix-core element kernels (FP32, p = 3, 4); returns hipErrorInvalidValue
-// for unsupported (N, p)
-hipError_t launch_stage_mfma(int N, int p, const StageArgs<float>& a, dim3 grid, hipStream_t st);
-bool mfma_supported(int N, int p);
-// elements per workgroup of the matrix-core kernel
+// Extra per-solve data of the matrix-core kernel: per (element, face) a
+// 16-bit word {bits 0..11 LDS slot of the neighbour (own tile element or
+// halo entry), bits 12..13 neighbour's local face, bit 14 left element},
+// stored [3][Kp]; per tile the halo element list [ntiles][hcap].
+struct MfmaArgs {
+    const uint16_t* fword;
+    const int* halo;
+    int hcap;
+    int pad;
+};
+
+// matrix-core element kernel (FP32, p = 3, 4): 64-element tiles, 4
+// wavefronts per workgroup, persistent over the tiles of its XCD's range
+// when the grid is smaller than the tile count; returns
+// hipErrorInvalidValue for unsupported (N, p)
 constexpr int MFMA_TILE_ELEMS = 64;
+constexpr int MFMA_MAX_HALO = 64;
+hipError_t launch_stage_mfma(int N, int p, const StageArgs<float>& a, const MfmaArgs& mx, dim3 grid, hipStream_t st);
+bool mfma_supported(int N, int p);
+size_t mfma_lds_bytes(int N, int hcap);
+int mfma_resident_groups(int N, int p, int hcap);
+hipError_t mfma_layout_probe(float* host_out);
 constexpr int VALU_TILE_ELEMS = 128;
 
+// resets ctrl->lam_bits and accumulates the state's maximum cell wave speed into it
 template <typename R>
-hipError_t launch_lambda(int N, const R* u, int K, int Kp, int NP, int tile_elems, int ntiles, int xcd_remap,
-                         double b0, const PhysP& ph, double* lam_part, dim3 grid, hipStream_t st);
-hipError_t launch_dt(Ctrl* ctrl, const double* lam_part, hipStream_t st);
+hipError_t launch_lambda(int N, const R* u, int K, int Kp, int NP, double b0, const PhysP& ph, Ctrl* ctrl,
+                         hipStream_t st);
+hipError_t launch_dt(Ctrl* ctrl, hipStream_t st);
 hipError_t launch_set_dt(Ctrl* ctrl, double dt, int advance, hipStream_t st);
 template <typename R>
 hipError_t launch_project_ic(int N, int p, R* c, const double* V, int K, int Kp, const IcP& ic, hipStream_t st);

@@ -13,12 +13,13 @@
 //   RK stage update    (time_integrator.cu:194-218, 445-464)
 //   CFL wave speeds    (euler.cu:252-278) of the new state on the last stage,
 //                      reduced per workgroup with wavefront shuffles.
+#include <algorithm>
+
 #include "dg_kernels.h"
 
 namespace dgmi {
 
-__constant__ TabAll<double> c_tab64;
-__constant__ TabAll<float> c_tab32;
+DGMI_TABLE_UPLOADER(upload_tables_kernels)
 
 // phi on face f at point index j where j is a runtime value in [0, NQ1)
 // given as the compile-time pair (j, NQ1-1-j) selected by `rev`
@@ -184,41 +185,37 @@ __global__ void __launch_bounds__(128) k_stage_valu(StageArgs<R> a) {
     }
     if (kind == ST_RK4_4 || kind == ST_RK2_2) {
         const double mx = block_max(lam, sh);
-        if (threadIdx.x == 0) a.lam_part[tile] = mx;
+        if (threadIdx.x == 0) lam_commit(mx, a.lam_bits);
     }
 }
 
-// Wave-speed partials of a state without stepping it (first step of a run,
-// dg_cfl_dt).  Same tile geometry as the stage kernel it precedes.
+// Maximum cell wave speed of a state without stepping it (first step of a
+// run, dg_cfl_dt): grid-stride over elements, workgroup max, one atomic max.
 template <int N, typename R>
-__global__ void __launch_bounds__(256) k_lambda(const R* u, int K, int Kp, int NP, int tile_elems, int ntiles,
-                                               int xcd_remap, double b0, PhysP ph, double* lam_part) {
+__global__ void __launch_bounds__(256) k_lambda(const R* u, int K, int Kp, int NP, double b0, PhysP ph,
+                                               unsigned long long* lam_bits) {
     __shared__ double sh[8];
-    const int tile = tile_of_block(blockIdx.x, gridDim.x, xcd_remap);
-    if (tile >= ntiles) return;
     double mx = 0.0;
-    for (int i = threadIdx.x; i < tile_elems; i += blockDim.x) {
-        const int e = tile * tile_elems + i;
-        if (e < K) {
-            double U[4];
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K; e += gridDim.x * blockDim.x) {
+        double U[4];
 #pragma unroll
-            for (int v = 0; v < N; v++) U[v] = (double)u[(size_t)(v * NP) * Kp + e] * b0;
-            mx = fmax(mx, cell_speed<N>(ph, U));
-        }
+        for (int v = 0; v < N; v++) U[v] = (double)u[(size_t)(v * NP) * Kp + e] * b0;
+        mx = fmax(mx, cell_speed<N>(ph, U));
     }
     mx = block_max(mx, sh);
-    if (threadIdx.x == 0) lam_part[tile] = mx;
+    if (threadIdx.x == 0) lam_commit(mx, lam_bits);
+}
+
+__global__ void k_reset_lam(Ctrl* ctrl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctrl->lam_bits = 0ull;
 }
 
 // dt = cfl * min_r / max lambda / (2p+1), clamped to t_end
-// (time_integrator.cu:254-270); one workgroup.
-__global__ void __launch_bounds__(256) k_dt(Ctrl* ctrl, const double* lam_part) {
-    __shared__ double sh[8];
-    const int n = ctrl->nparts;
-    double mx = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = fmax(mx, lam_part[i]);
-    mx = block_max(mx, sh);
-    if (threadIdx.x == 0) {
+// (time_integrator.cu:254-270); reads and clears the accumulated maximum.
+__global__ void k_dt(Ctrl* ctrl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double mx = __longlong_as_double((long long)ctrl->lam_bits);
+        ctrl->lam_bits = 0ull;
         if (ctrl->done) {
             ctrl->active = 0;
             return;
@@ -249,6 +246,7 @@ __global__ void k_set_dt(Ctrl* ctrl, double dt, int advance) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctrl->dt = dt;
         ctrl->active = 1;
+        ctrl->lam_bits = 0ull;
         if (advance) {
             ctrl->t += dt;
             ctrl->steps += 1;
@@ -393,23 +391,23 @@ template hipError_t launch_stage_valu<float>(int, int, const StageArgs<float>&, 
 template hipError_t launch_stage_valu<double>(int, int, const StageArgs<double>&, dim3, hipStream_t);
 
 template <typename R>
-hipError_t launch_lambda(int N, const R* u, int K, int Kp, int NP, int tile_elems, int ntiles, int xcd_remap,
-                         double b0, const PhysP& ph, double* lam_part, dim3 grid, hipStream_t st) {
+hipError_t launch_lambda(int N, const R* u, int K, int Kp, int NP, double b0, const PhysP& ph, Ctrl* ctrl,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_lam, dim3(1), dim3(64), 0, st, ctrl);
+    const dim3 grid(std::min((K + 255) / 256, 2048));
+    unsigned long long* bits = &ctrl->lam_bits;
     if (N == 1)
-        hipLaunchKernelGGL((k_lambda<1, R>), grid, dim3(256), 0, st, u, K, Kp, NP, tile_elems, ntiles, xcd_remap, b0, ph,
-                           lam_part);
+        hipLaunchKernelGGL((k_lambda<1, R>), grid, dim3(256), 0, st, u, K, Kp, NP, b0, ph, bits);
     else
-        hipLaunchKernelGGL((k_lambda<4, R>), grid, dim3(256), 0, st, u, K, Kp, NP, tile_elems, ntiles, xcd_remap, b0, ph,
-                           lam_part);
+        hipLaunchKernelGGL((k_lambda<4, R>), grid, dim3(256), 0, st, u, K, Kp, NP, b0, ph, bits);
     return hipGetLastError();
 }
-template hipError_t launch_lambda<float>(int, const float*, int, int, int, int, int, int, double, const PhysP&,
-                                         double*, dim3, hipStream_t);
-template hipError_t launch_lambda<double>(int, const double*, int, int, int, int, int, int, double, const PhysP&,
-                                          double*, dim3, hipStream_t);
+template hipError_t launch_lambda<float>(int, const float*, int, int, int, double, const PhysP&, Ctrl*, hipStream_t);
+template hipError_t launch_lambda<double>(int, const double*, int, int, int, double, const PhysP&, Ctrl*,
+                                          hipStream_t);
 
-hipError_t launch_dt(Ctrl* ctrl, const double* lam_part, hipStream_t st) {
-    hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, st, ctrl, lam_part);
+hipError_t launch_dt(Ctrl* ctrl, hipStream_t st) {
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, st, ctrl);
     return hipGetLastError();
 }
 
@@ -493,9 +491,28 @@ hipError_t upload_tables() {
     static TabAll<float> h32;
     fill_all<double>(h64);
     fill_all<float>(h32);
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab64), &h64, sizeof(h64));
-    if (e != hipSuccess) return e;
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &h32, sizeof(h32));
+    hipError_t e = upload_tables_kernels(h64, h32);
+    if (e == hipSuccess) e = upload_tables_mfma(h64, h32);
+    if (e == hipSuccess) e = upload_tables_batch(h64, h32);
+    return e;
+}
+
+}  // namespace dgmi
+
+namespace dgmi {
+
+// Streaming copy (16-byte vector loads and stores), the bandwidth reference
+// for the roofline and the scale of the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
+    const int64_t n = bytes / 16;
+    hipLaunchKernelGGL(k_stream_copy, dim3(2048), dim3(256), 0, st, (const float4*)src, (float4*)dst, n);
+    return hipGetLastError();
 }
 
 }  // namespace dgmi

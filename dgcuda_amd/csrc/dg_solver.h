@@ -36,7 +36,8 @@ Status check_problem(const dg_problem_desc& d, int p, int precision);
 
 // Kernel selection and launch shape of the stage kernels.
 struct LaunchShape {
-    int kind = 1;        // 1 = VALU (lane per element), 2 = matrix cores
+    int kind = 1;        // 1 = VALU (lane per element), 2 / 3 = matrix cores, 64 / 128-element tiles
+    int sub = 0;         // matrix cores: 16-element sub-tiles per wavefront
     int tile_elems = 0;  // elements per workgroup
     int ntiles = 0;      // workgroups carrying elements
     int grid = 0;        // workgroups launched (multiple of 8 when remapped)

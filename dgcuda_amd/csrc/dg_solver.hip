@@ -101,7 +101,11 @@ class SolverT : public Solver {
     R *c = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr;
     R *geo_e = nullptr, *geo_f = nullptr;
     int* nbr = nullptr;
-    double *V = nullptr, *Jd = nullptr, *lam_part = nullptr, *audit_part = nullptr;
+    std::vector<int> h_nbr[3];
+    uint16_t* fword = nullptr;
+    int* halo = nullptr;
+    MfmaArgs mx{};
+    double *V = nullptr, *Jd = nullptr, *audit_part = nullptr;
     Ctrl* ctrl = nullptr;
     Ctrl hctrl{};
     LaunchShape shape;
@@ -111,11 +115,11 @@ class SolverT : public Solver {
     int graph_scheme = 0;
     int64_t bytes = 0;
     int audit_blocks = 0;
-    int lam_capacity = 0;
+    double b0 = 0;  // phi_0 at the first volume point (cell mean = c_0 * b0)
 
     ~SolverT() override {
         if (gexec) hipGraphExecDestroy(gexec);
-        void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, lam_part, audit_part, ctrl};
+        void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, audit_part, ctrl, fword, halo};
         for (void* b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
@@ -142,6 +146,7 @@ class SolverT : public Solver {
         precision = sizeof(R) == 8 ? DG_FP64 : DG_FP32;
         phys = make_phys(pr);
         ic = make_ic(pr, m.period_x, m.period_y);
+        b0 = build_tables(p).basis[0];
         if (m.ordered) {
             perm.resize(K);
             for (int i = 0; i < K; i++) perm[i] = i;
@@ -167,8 +172,6 @@ class SolverT : public Solver {
         if (!(st = alloc((void**)&nbr, 3 * (size_t)Kp * sizeof(int)))) return st;
         if (!(st = alloc((void**)&V, 6 * (size_t)Kp * sizeof(double)))) return st;
         if (!(st = alloc((void**)&Jd, (size_t)Kp * sizeof(double)))) return st;
-        lam_capacity = Kp / 64 + 16;
-        if (!(st = alloc((void**)&lam_part, (size_t)lam_capacity * sizeof(double)))) return st;
         audit_blocks = (K + 255) / 256;
         if (!(st = alloc((void**)&audit_part, (size_t)audit_blocks * 8 * sizeof(double)))) return st;
         if (!(st = alloc((void**)&ctrl, sizeof(Ctrl)))) return st;
@@ -199,6 +202,7 @@ class SolverT : public Solver {
         HIPST(hipMemcpyAsync(geo_e, he.data(), he.size() * sizeof(R), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(geo_f, hf.data(), hf.size() * sizeof(R), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(nbr, hn.data(), hn.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        for (int f = 0; f < 3; f++) h_nbr[f] = g.nbr[f];
         HIPST(hipMemcpyAsync(V, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(Jd, hj.data(), hj.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         hctrl = Ctrl{};
@@ -210,22 +214,86 @@ class SolverT : public Solver {
         return Status::ok();
     }
 
+    // Matrix-core tiling: per (element, face) the LDS slot of the neighbour
+    // (inside the tile, or in the tile's halo list), and the halo lists.
+    Status build_mfma_tiles(int tile_elems) {
+        const int ntiles = (K + tile_elems - 1) / tile_elems;
+        std::vector<uint16_t> words(3 * (size_t)Kp, 0);
+        std::vector<std::vector<int>> halos(ntiles);
+        int hmax = 0;
+        for (int t = 0; t < ntiles; t++) {
+            const int b = t * tile_elems, end = std::min(b + tile_elems, K);
+            std::vector<int>& h = halos[t];
+            for (int e = b; e < end; e++)
+                for (int f = 0; f < 3; f++) {
+                    const int wv = h_nbr[f][e];
+                    const int nb = wv & NB_ELEM_MASK, fn = (wv >> 28) & 3, left = (wv >> 30) & 1;
+                    int slot;
+                    if (nb >= b && nb < end) {
+                        slot = nb - b;
+                    } else {
+                        auto it = std::find(h.begin(), h.end(), nb);
+                        slot = tile_elems + (int)(it - h.begin());
+                        if (it == h.end()) h.push_back(nb);
+                    }
+                    if (slot >= 4096) return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
+                    words[f * (size_t)Kp + e] = (uint16_t)(slot | (fn << 12) | (left << 14));
+                }
+            hmax = std::max(hmax, (int)h.size());
+        }
+        const int hcap = std::max(16, (hmax + 15) / 16 * 16);
+        if (hcap > MFMA_MAX_HALO || mfma_lds_bytes(N, hcap) > 65536)
+            return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large for the matrix-core kernel");
+        std::vector<int> hl((size_t)ntiles * hcap);
+        for (int t = 0; t < ntiles; t++)
+            for (int i = 0; i < hcap; i++)
+                hl[(size_t)t * hcap + i] = i < (int)halos[t].size() ? halos[t][i] : t * tile_elems;
+        if (fword) hipFree(fword);
+        if (halo) hipFree(halo);
+        fword = nullptr;
+        halo = nullptr;
+        HIPST(hipMalloc((void**)&fword, words.size() * sizeof(uint16_t)));
+        HIPST(hipMalloc((void**)&halo, hl.size() * sizeof(int)));
+        HIPST(hipMemcpyAsync(fword, words.data(), words.size() * sizeof(uint16_t), hipMemcpyHostToDevice, stream));
+        HIPST(hipMemcpyAsync(halo, hl.data(), hl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        HIPST(hipStreamSynchronize(stream));
+        mx.fword = fword;
+        mx.halo = halo;
+        mx.hcap = hcap;
+        return Status::ok();
+    }
+
     Status choose_shape(int req) {
         kernel_req = req;
         int kind = req;
-        if (kind == 0) kind = (sizeof(R) == 4 && mfma_supported(N, p)) ? 2 : 1;
-        if (kind == 2 && !(sizeof(R) == 4 && mfma_supported(N, p)))
+        const bool mf = sizeof(R) == 4 && mfma_supported(N, p);
+        if (kind == 0) kind = mf ? 3 : 1;
+        if ((kind == 2 || kind == 3) && !mf)
             return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs FP32 and p = 3 or 4");
-        if (kind != 1 && kind != 2) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1 or 2");
+        if (kind < 1 || kind > 3) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1, 2 or 3");
         LaunchShape s;
         s.kind = kind;
-        s.tile_elems = (kind == 2) ? MFMA_TILE_ELEMS : VALU_TILE_ELEMS;
+        s.tile_elems = (kind == 1) ? VALU_TILE_ELEMS : MFMA_TILE_ELEMS;
         s.ntiles = (K + s.tile_elems - 1) / s.tile_elems;
         s.xcd_remap = 1;
         s.grid = (s.ntiles + 7) / 8 * 8;
-        if (s.ntiles > lam_capacity) return Status::err(DG_ERR_ARG, "internal: wave-speed partial buffer too small");
+        if (kind >= 2) {
+            Status st = build_mfma_tiles(s.tile_elems);
+            if (st && kind == 2) {
+                // persistent: as many workgroups as fit on the device at once
+                const int res = mfma_resident_groups(N, p, mx.hcap);
+                if (res >= 8) s.grid = std::min(s.grid, res / 8 * 8);
+            }
+            if (!st) {
+                if (req != 0) return st;
+                // automatic choice falls back to the VALU kernel
+                s.kind = 1;
+                s.tile_elems = VALU_TILE_ELEMS;
+                s.ntiles = (K + s.tile_elems - 1) / s.tile_elems;
+                s.grid = (s.ntiles + 7) / 8 * 8;
+            }
+        }
         shape = s;
-        hctrl.nparts = s.ntiles;
         lam_valid = false;
         if (gexec) {
             hipGraphExecDestroy(gexec);
@@ -255,7 +323,7 @@ class SolverT : public Solver {
         a.geo_f = geo_f;
         a.nbr = nbr;
         a.ctrl = ctrl;
-        a.lam_part = lam_part;
+        a.lam_bits = &ctrl->lam_bits;
         a.K = K;
         a.Kp = Kp;
         a.kind = kind;
@@ -264,8 +332,8 @@ class SolverT : public Solver {
         a.phys = phys;
         hipError_t e;
         if constexpr (sizeof(R) == 4) {
-            if (shape.kind == 2)
-                e = launch_stage_mfma(N, p, a, dim3(shape.grid), stream);
+            if (shape.kind >= 2)
+                e = launch_stage_mfma(N, p, a, mx, dim3(shape.grid), stream);
             else
                 e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), stream);
         } else {
@@ -276,9 +344,7 @@ class SolverT : public Solver {
 
     Status ensure_lambda() {
         if (lam_valid) return Status::ok();
-        const double b0 = build_tables(p).basis[0];
-        HIPST(launch_lambda<R>(N, c, K, Kp, NP, shape.tile_elems, shape.ntiles, shape.xcd_remap, b0, phys, lam_part,
-                               dim3(shape.grid), stream));
+        HIPST(launch_lambda<R>(N, c, K, Kp, NP, b0, phys, ctrl, stream));
         lam_valid = true;
         return Status::ok();
     }
@@ -305,7 +371,7 @@ class SolverT : public Solver {
                 }
                 hipGraph_t g = nullptr;
                 HIPST(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                Status st = hip_status(launch_dt(ctrl, lam_part, stream), "dt kernel");
+                Status st = hip_status(launch_dt(ctrl, stream), "dt kernel");
                 if (st) st = enqueue_stages(scheme);
                 hipError_t ee = hipStreamEndCapture(stream, &g);
                 if (!st) {
@@ -321,7 +387,7 @@ class SolverT : public Solver {
             HIPST(hipGraphLaunch(gexec, stream));
             return Status::ok();
         }
-        HIPST(launch_dt(ctrl, lam_part, stream));
+        HIPST(launch_dt(ctrl, stream));
         return enqueue_stages(scheme);
     }
 
@@ -335,7 +401,6 @@ class SolverT : public Solver {
         hctrl.cfl = cfl;
         hctrl.min_r = min_r;
         hctrl.two_p1 = 2. * p + 1.;
-        hctrl.nparts = shape.ntiles;
         Status st = push_ctrl();
         if (!st) return st;
         return ensure_lambda();
@@ -391,6 +456,7 @@ class SolverT : public Solver {
         Status st = upload_ref(u_ref, ub);
         if (!st) return st;
         HIPST(launch_set_dt(ctrl, dt, 0, stream));
+        lam_valid = false;  // set_dt cleared the accumulated wave speed
         if (!(st = stage(ST_RHS, ub, ub, ua))) return st;
         return download_ref(ua, k_ref);
     }
@@ -410,11 +476,11 @@ class SolverT : public Solver {
     Status cfl_dt(double cfl, double* dt) override {
         Status st = ensure_lambda();
         if (!st) return st;
-        std::vector<double> h(shape.ntiles);
-        HIPST(hipMemcpyAsync(h.data(), lam_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        unsigned long long bits = 0;
+        HIPST(hipMemcpyAsync(&bits, &ctrl->lam_bits, sizeof(bits), hipMemcpyDeviceToHost, stream));
         HIPST(hipStreamSynchronize(stream));
-        double mx = 0;
-        for (double x : h) mx = std::max(mx, x);
+        double mx;
+        std::memcpy(&mx, &bits, sizeof(mx));
         *dt = cfl * min_r / mx / (2. * p + 1.);
         return Status::ok();
     }
@@ -453,7 +519,6 @@ class SolverT : public Solver {
     }
 
     Status audit(dg_audit_out* out) override {
-        const double b0 = build_tables(p).basis[0];
         HIPST(launch_audit<R>(N, c, Jd, K, Kp, NP, b0, phys.gamma, audit_part, audit_blocks, stream));
         std::vector<double> h((size_t)audit_blocks * 8);
         HIPST(hipMemcpyAsync(h.data(), audit_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -506,7 +571,7 @@ class SolverT : public Solver {
         HIPST(hipEventRecord(ev[2 * nst + 2 * nsteps], stream));
         for (int s = 0; s < nsteps; s++) {
             HIPST(hipEventRecord(ev[2 * nst + 2 * s], stream));
-            HIPST(launch_dt(ctrl, lam_part, stream));
+            HIPST(launch_dt(ctrl, stream));
             HIPST(hipEventRecord(ev[2 * nst + 2 * s + 1], stream));
             const int kinds[4] = {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4};
             const R* ins[4] = {c, ua, ub, ua};

@@ -163,7 +163,8 @@ int dg_audit(dg_solver* s, dg_audit_out* out);
 int dg_eval_vertices(dg_solver* s, int var, double* uv);
 int dg_solver_sync(dg_solver* s);
 /* kernel selection for the stage kernels: 0 = automatic (tuned table),
- * 1 = one element per lane (VALU), 2 = matrix-core tiles (FP32, p = 3, 4) */
+ * 1 = one element per lane (VALU), 2 / 3 = matrix-core tiles of 64 / 128
+ * elements (FP32, p = 3, 4) */
 int dg_solver_set_kernel(dg_solver* s, int kind);
 int dg_solver_get_kernel(const dg_solver* s, int* kind);
 /* Times nsteps RK4 steps with HIP events on the solver's stream, one event
@@ -173,6 +174,15 @@ int dg_solver_get_kernel(const dg_solver* s, int* kind);
 int dg_time_stage_kernels(dg_solver* s, int nsteps, double cfl, double* out);
 /* device buffer bytes held by the solver */
 int dg_solver_device_bytes(const dg_solver* s, int64_t* bytes);
+/* Device-to-device streaming copy of `bytes` (multiple of 16), iters times,
+ * timed with HIP events: out[0] = mean ms per copy, out[1] = GB/s counting
+ * read + write bytes.  The bandwidth reference of the roofline (no
+ * counterpart in the reference). */
+int dg_bandwidth_probe(int64_t bytes, int iters, double* out);
+/* Runs one 16x16x4 FP32 matrix-core product on the GPU and compares it with
+ * the host product under the lane layout the kernels assume; *max_err = max
+ * absolute difference (no counterpart in the reference). */
+int dg_selftest_mfma(double* max_err);
 
 /* ---------------- many solves, one persistent launch per RK stage ---------------- */
 int dg_batch_create(int precision, dg_batch** out);

@@ -26,5 +26,23 @@ def perturbed_state(o, prob, period, seed=0, amp=1e-2):
     return u + amp * scale * rng.standard_normal(u.shape)
 
 
+def shuffled_mesh(n, seed=7, lo=0.0, hi=10.0):
+    """A jittered periodic mesh with elements and sides renumbered at random
+    (so the solver's Morton reordering and irregular tile halos are used)."""
+    m = mesh(n, lo, hi)
+    V, es, sd, si = m.arrays()
+    K, S = V.shape[1], sd.shape[1]
+    rng = np.random.default_rng(seed)
+    pe, ps = rng.permutation(K), rng.permutation(S)   # new -> old
+    ie, is_ = np.argsort(pe), np.argsort(ps)          # old -> new
+    V2 = V[:, pe]
+    es2 = is_[es[:, pe]].astype(np.int32)
+    sd2 = sd[:, ps]
+    si2 = si[:, ps].copy()
+    si2[0] = ie[si2[0]]
+    si2[1] = ie[si2[1]]
+    return dg.Mesh.from_arrays(V2, es2, sd2, si2, period=(hi - lo, hi - lo))
+
+
 def rel_max(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))

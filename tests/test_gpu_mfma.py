@@ -1,0 +1,59 @@
+"""Matrix-core (FP32, p = 3, 4) stage kernels against the FP64 oracle and the
+VALU kernel.  FP32 tolerance: 2e-5 relative max-norm for one RHS evaluation,
+1e-5 relative L2 after a run."""
+import numpy as np
+import pytest
+
+import dgcuda_amd as dg
+import oracle as orc
+from helpers import mesh, perturbed_state, problem_for, rel_max, shuffled_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_lane_layout():
+    assert dg.selftest_mfma() == 0.0
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+@pytest.mark.parametrize("p", [3, 4])
+@pytest.mark.parametrize("N", [1, 4])
+def test_mfma_rhs_parity(N, p, kind):
+    print("mfma rhs N=%d p=%d kind=%d" % (N, p, kind), flush=True)
+    for m in (mesh(12), shuffled_mesh(12)):
+        prob = problem_for(N)
+        o = orc.Oracle.from_mesh(m, p, prob)
+        u = perturbed_state(o, prob, m.period, seed=p)
+        ref = o.rhs(u, dt=0.01)
+        s = dg.Solver(m, prob, p, dg.DG_FP32)
+        s.set_kernel(kind)
+        assert s.kernel() == kind
+        err = rel_max(s.rhs(u, dt=0.01), ref)
+        print("  rel max err %.3e" % err)
+        assert err <= 2e-5
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+def test_mfma_run_vs_oracle(kind):
+    m = shuffled_mesh(24, seed=3)
+    prob = dg.euler_vortex(((3.0, 4.0, 6.0), (2.5, 7.0, 3.0)))
+    o = orc.Oracle.from_mesh(m, 4, prob)
+    u0 = o.project_ic(orc.ic_array(prob, m.period))
+    ref, n_ref, _ = o.integrate(u0, 0.25, 0.3)
+    s = dg.Solver(m, prob, 4, dg.DG_FP32)
+    s.set_kernel(kind)
+    s.project_ic()
+    n, t = s.integrate(0.25, 0.3)
+    err = orc.rel_l2(s.get_state(), ref)
+    print("kind %d steps %d/%d rel L2 %.3e" % (kind, n, n_ref, err))
+    assert t == 0.25 and err <= 1e-5
+    a0 = o.mass(u0)
+    a1 = s.audit()["mass"]
+    assert np.all(np.abs(np.array(a1) - a0) <= 2e-6 * np.abs(a0) + 1e-9)
+
+
+def test_mfma_auto_selected_for_fp32_p4():
+    s = dg.Solver(mesh(8), problem_for(4), 4, dg.DG_FP32)
+    assert s.kernel() in (2, 3)
+    s64 = dg.Solver(mesh(8), problem_for(4), 4, dg.DG_FP64)
+    assert s64.kernel() == 1
