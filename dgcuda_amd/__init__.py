@@ -20,7 +20,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdgmi.so")
+# DGMI_LIB selects another build of the library (kernel-variant comparisons)
+LIB_PATH = os.environ.get("DGMI_LIB") or os.path.join(_HERE, "libdgmi.so")
 
 DG_OK = 0
 DG_PHYS_ADVECTION, DG_PHYS_EULER = 1, 4

@@ -38,32 +38,37 @@ def _run(cmd):
     return p.stdout
 
 
-def _compile(src):
-    path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, src.replace(".", "_") + ".o")
-    if src in GXX_SOURCES:
-        cmd = ["g++"] + GXX_FLAGS + ["-c", path, "-o", obj]
-    else:
-        cmd = [HIPCC] + HIP_FLAGS + ["-x", "hip", "-c", path, "-o", obj]
-    _run(cmd)
-    return obj
-
-
-def build(verbose=False):
-    """Compile every source and link dgcuda_amd/libdgmi.so (always rebuilds)."""
+def build(verbose=False, out=None, defines=()):
+    """Compile every source and link dgcuda_amd/libdgmi.so (always rebuilds).
+    `out` and `defines` (extra -D flags for the HIP sources) build a variant
+    library elsewhere for kernel comparisons."""
     if not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at %s" % HIPCC)
-    os.makedirs(OBJ, exist_ok=True)
+    lib = out or LIB
+    objdir = OBJ if not out else out + ".objs"
+    os.makedirs(objdir, exist_ok=True)
+    extra = ["-D" + d for d in defines]
+
+    def comp(src):
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(objdir, src.replace(".", "_") + ".o")
+        if src in GXX_SOURCES:
+            cmd = ["g++"] + GXX_FLAGS + ["-c", path, "-o", obj]
+        else:
+            cmd = [HIPCC] + HIP_FLAGS + extra + ["-x", "hip", "-c", path, "-o", obj]
+        _run(cmd)
+        return obj
+
     srcs = GXX_SOURCES + HIP_SOURCES
     jobs = min(len(srcs), max(1, (os.cpu_count() or 4)), 8)
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(comp, srcs))
+    tmp = lib + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib)
+    return lib
 
 
 def clean():

@@ -96,6 +96,36 @@ __device__ __forceinline__ void lam_commit(double mx, unsigned long long* bits) 
     if (mx > 0.0) atomicMax(bits, (unsigned long long)__double_as_longlong(mx));
 }
 
+// Start of a step on the device: dt = cfl * min_r / max lambda / (2p+1),
+// clamped to t_end, t advanced before the stages (time_integrator.cu:254-270);
+// reads and clears the accumulated maximum wave speed.  One thread.
+__device__ __forceinline__ void dt_update(Ctrl* ctrl) {
+    const double mx = __longlong_as_double((long long)ctrl->lam_bits);
+    ctrl->lam_bits = 0ull;
+    if (ctrl->done) {
+        ctrl->active = 0;
+        return;
+    }
+    double t = ctrl->t;
+    if (ctrl->has_end && !(t < ctrl->t_end)) {
+        ctrl->active = 0;
+        ctrl->done = 1;
+        ctrl->dt = 0.0;
+        return;
+    }
+    double dt = ctrl->cfl * ctrl->min_r / mx / ctrl->two_p1;
+    if (ctrl->has_end && t + dt > ctrl->t_end) {
+        dt = ctrl->t_end - t;
+        t = ctrl->t_end;
+    } else {
+        t += dt;
+    }
+    ctrl->dt = dt;
+    ctrl->t = t;
+    ctrl->steps += 1;
+    ctrl->active = 1;
+}
+
 struct PhysP {
     double gamma;
     double ax, ay;
@@ -231,7 +261,9 @@ __device__ __forceinline__ R llf_speed(const PhysP& ph, const R* UL, const R* UR
             R ul = UL[1] * il, vl = UL[2] * il, ur = UR[1] * ir, vr = UR[2] * ir;
             R pl = (g - R(1)) * (UL[3] - R(0.5) * (UL[1] * ul + UL[2] * vl));
             R pr = (g - R(1)) * (UR[3] - R(0.5) * (UR[1] * ur + UR[2] * vr));
-            R cl = __builtin_sqrtf(g * pl * il), cr = __builtin_sqrtf(g * pr * ir);
+            // hardware square root (about 1 ulp): the FP32 path does not
+            // reproduce the reference's rounding bit for bit
+            R cl = __builtin_amdgcn_sqrtf(g * pl * il), cr = __builtin_amdgcn_sqrtf(g * pr * ir);
             R lm = fabsf(nx * ul + ny * vl) + cl;
             R rm = fabsf(nx * ur + ny * vr) + cr;
             return fmaxf(lm, rm);

@@ -15,174 +15,20 @@
 //                      reduced per workgroup with wavefront shuffles.
 #include <algorithm>
 
-#include "dg_kernels.h"
+#include "dg_valu_body.h"
 
 namespace dgmi {
 
 DGMI_TABLE_UPLOADER(upload_tables_kernels)
 
-// phi on face f at point index j where j is a runtime value in [0, NQ1)
-// given as the compile-time pair (j, NQ1-1-j) selected by `rev`
-template <typename R, int P>
-__device__ __forceinline__ R face_phi(const TabP<R, P>& T, int f, int m, int j, bool rev) {
-    constexpr int NQ1 = P + 1;
-    return rev ? T.S[f][m][NQ1 - 1 - j] : T.S[f][m][j];
-}
-
 template <int N, int P, typename R>
 __global__ void __launch_bounds__(128) k_stage_valu(StageArgs<R> a) {
-    constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1;
-    const TabP<R, P>& T = tab<R, P>();
     __shared__ double sh[8];
     if (!a.ctrl->active) return;
     const int tile = tile_of_block(blockIdx.x, gridDim.x, a.xcd_remap);
     if (tile >= a.ntiles) return;
-    const int e = tile * blockDim.x + threadIdx.x;
-    const bool valid = e < a.K;
-    const int ee = valid ? e : a.K - 1;
-    const int Kp = a.Kp;
-    const R dt = (R)a.ctrl->dt;
-
-    R u[N][NP];
-#pragma unroll
-    for (int v = 0; v < N; v++)
-#pragma unroll
-        for (int m = 0; m < NP; m++) u[v][m] = a.u[(size_t)(v * NP + m) * Kp + ee];
-
-    R tot[N][NP];
-#pragma unroll
-    for (int v = 0; v < N; v++)
-#pragma unroll
-        for (int m = 0; m < NP; m++) tot[v][m] = R(0);
-
-    // ---- volume: sum_q F(U_q) . (metric * grad phi_m) ----
-    {
-        const R xr = a.geo_e[ee], yr = a.geo_e[Kp + ee], xs = a.geo_e[2 * Kp + ee], ys = a.geo_e[3 * Kp + ee];
-#pragma unroll 2
-        for (int q = 0; q < NQ; q++) {
-            R U[N];
-#pragma unroll
-            for (int v = 0; v < N; v++) U[v] = R(0);
-#pragma unroll
-            for (int m = 0; m < NP; m++)
-#pragma unroll
-                for (int v = 0; v < N; v++) U[v] += u[v][m] * T.B[m][q];
-            R fx[N], fy[N];
-            flux<N, R>(a.phys, U, fx, fy);
-#pragma unroll
-            for (int m = 0; m < NP; m++) {
-                const R bgx = T.Gx[m][q], bgy = T.Gy[m][q];
-#pragma unroll
-                for (int v = 0; v < N; v++) tot[v][m] += fx[v] * (bgx * ys - bgy * yr) + fy[v] * (-bgx * xs + bgy * xr);
-            }
-        }
-    }
-
-    // ---- the three faces, added in face order as eval_rhs adds s1, s2, s3 ----
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-        const int w = a.nbr[f * Kp + ee];
-        const int nb = w & NB_ELEM_MASK;
-        const int fn = (w >> 28) & 3;
-        const bool left = (w >> 30) & 1;
-        const R nx = a.geo_f[(3 * f + 0) * Kp + ee];
-        const R ny = a.geo_f[(3 * f + 1) * Kp + ee];
-        const R len = a.geo_f[(3 * f + 2) * Kp + ee];
-        R FS[NQ1][N];
-#pragma unroll
-        for (int j = 0; j < NQ1; j++) {
-            // canonical point j of the side: the left element's trace at its
-            // point j, the right element's at NQ1-1-j
-            R Uo[N], Un[N];
-#pragma unroll
-            for (int v = 0; v < N; v++) {
-                Uo[v] = R(0);
-                Un[v] = R(0);
-            }
-#pragma unroll
-            for (int m = 0; m < NP; m++) {
-                const R so = face_phi<R, P>(T, f, m, j, !left);
-                const R s0 = face_phi<R, P>(T, 0, m, j, left);
-                const R s1 = face_phi<R, P>(T, 1, m, j, left);
-                const R s2 = face_phi<R, P>(T, 2, m, j, left);
-                const R sn = (fn == 0) ? s0 : ((fn == 1) ? s1 : s2);
-#pragma unroll
-                for (int v = 0; v < N; v++) {
-                    Uo[v] += u[v][m] * so;
-                    Un[v] += a.u[(size_t)(v * NP + m) * Kp + nb] * sn;
-                }
-            }
-            R UL[N], UR[N];
-#pragma unroll
-            for (int v = 0; v < N; v++) {
-                UL[v] = left ? Uo[v] : Un[v];
-                UR[v] = left ? Un[v] : Uo[v];
-            }
-            R fxl[N], fyl[N], fxr[N], fyr[N];
-            flux<N, R>(a.phys, UL, fxl, fyl);
-            flux<N, R>(a.phys, UR, fxr, fyr);
-            const R lam = llf_speed<N, R>(a.phys, UL, UR, nx, ny);
-#pragma unroll
-            for (int v = 0; v < N; v++)
-                FS[j][v] = T.W1[j] * (R(0.5) * ((fxl[v] + fxr[v]) * nx + (fyl[v] + fyr[v]) * ny + lam * (UL[v] - UR[v])));
-        }
-        const R coef = left ? -len / R(2) : len / R(2);
-#pragma unroll
-        for (int m = 0; m < NP; m++)
-#pragma unroll
-            for (int v = 0; v < N; v++) {
-                R s = R(0);
-#pragma unroll
-                for (int j = 0; j < NQ1; j++) s += FS[j][v] * face_phi<R, P>(T, f, m, j, !left);
-                tot[v][m] += coef * s;
-            }
-    }
-
-    // ---- inverse mass (J*I) and the RK stage update ----
-    const R J = a.geo_e[4 * Kp + ee];
-    const R cj = R(1) / J * dt;
     const int kind = a.kind;
-    double lam = 0.0;
-    if (kind == ST_RHS) {
-        if (valid)
-#pragma unroll
-            for (int v = 0; v < N; v++)
-#pragma unroll
-                for (int m = 0; m < NP; m++) a.out[(size_t)(v * NP + m) * Kp + e] = cj * tot[v][m];
-    } else {
-        R c0[N];
-#pragma unroll
-        for (int v = 0; v < N; v++)
-#pragma unroll
-            for (int m = 0; m < NP; m++) {
-                const size_t ix = (size_t)(v * NP + m) * Kp + ee;
-                const R k = cj * tot[v][m];
-                const R c = (kind == ST_RK4_1 || kind == ST_RK2_1) ? u[v][m] : a.c[ix];
-                R o;
-                if (kind == ST_RK4_1) {
-                    if (valid) a.acc[ix] = k / R(6);
-                    o = c + R(0.5) * k;
-                } else if (kind == ST_RK4_2 || kind == ST_RK4_3) {
-                    if (valid) a.acc[ix] = a.acc[ix] + k / R(3);
-                    o = (kind == ST_RK4_2) ? c + R(0.5) * k : c + R(1.0) * k;
-                } else if (kind == ST_RK4_4) {
-                    o = c + (a.acc[ix] + k / R(6));
-                } else if (kind == ST_RK2_1) {
-                    o = c + R(0.5) * k;
-                } else {
-                    o = c + k;
-                }
-                if (valid) a.out[ix] = o;
-                if (m == 0) c0[v] = o;
-            }
-        if (kind == ST_RK4_4 || kind == ST_RK2_2) {
-            double U[4];
-            const double b0 = tab<double, P>().B[0][0];
-#pragma unroll
-            for (int v = 0; v < N; v++) U[v] = (double)c0[v] * b0;
-            lam = valid ? cell_speed<N>(a.phys, U) : 0.0;
-        }
-    }
+    const double lam = valu_process<N, P, R>(a, tile * blockDim.x + threadIdx.x, kind);
     if (kind == ST_RK4_4 || kind == ST_RK2_2) {
         const double mx = block_max(lam, sh);
         if (threadIdx.x == 0) lam_commit(mx, a.lam_bits);
@@ -213,32 +59,7 @@ __global__ void k_reset_lam(Ctrl* ctrl) {
 // dt = cfl * min_r / max lambda / (2p+1), clamped to t_end
 // (time_integrator.cu:254-270); reads and clears the accumulated maximum.
 __global__ void k_dt(Ctrl* ctrl) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double mx = __longlong_as_double((long long)ctrl->lam_bits);
-        ctrl->lam_bits = 0ull;
-        if (ctrl->done) {
-            ctrl->active = 0;
-            return;
-        }
-        double t = ctrl->t;
-        if (ctrl->has_end && !(t < ctrl->t_end)) {
-            ctrl->active = 0;
-            ctrl->done = 1;
-            ctrl->dt = 0.0;
-            return;
-        }
-        double dt = ctrl->cfl * ctrl->min_r / mx / ctrl->two_p1;
-        if (ctrl->has_end && t + dt > ctrl->t_end) {
-            dt = ctrl->t_end - t;
-            t = ctrl->t_end;
-        } else {
-            t += dt;
-        }
-        ctrl->dt = dt;
-        ctrl->t = t;
-        ctrl->steps += 1;
-        ctrl->active = 1;
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) dt_update(ctrl);
 }
 
 // host-chosen step: activate the stages with the given dt

@@ -44,9 +44,29 @@ struct LaunchShape {
     int xcd_remap = 1;
 };
 
+// Device-side view of one solve, for the batched kernels.
+struct SlotInfo {
+    void *c, *acc, *ua, *ub, *geo_e, *geo_f;
+    const int* nbr;
+    const uint16_t* fword;
+    const int* halo;
+    Ctrl* ctrl;
+    int K, Kp, N, p, hcap, kind;
+    PhysP phys;
+};
+
 class Solver {
    public:
     virtual ~Solver() {}
+    virtual void slot_info(SlotInfo* out) = 0;
+    // pushes the run parameters to the control block and computes the wave
+    // speed of the resident state, then waits for the solver's stream
+    virtual Status begin(double t_end, bool has_end, double cfl) = 0;
+    // reads the control block back (t, steps) after work issued elsewhere
+    virtual Status refresh() = 0;
+    // the last stage of a step issued elsewhere left the wave speed in ctrl
+    virtual void set_lambda_valid(bool v) = 0;
+    virtual hipStream_t get_stream() const = 0;
     static Status create(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out);
 
     int K = 0, S = 0, Kp = 0, p = 0, NP = 0, NQ = 0, NQ1 = 0, N = 0, precision = DG_FP64;

@@ -267,9 +267,9 @@ class SolverT : public Solver {
         kernel_req = req;
         int kind = req;
         const bool mf = sizeof(R) == 4 && mfma_supported(N, p);
-        if (kind == 0) kind = mf ? 3 : 1;
+        if (kind == 0) kind = (mf && p >= 3) ? 2 : 1;
         if ((kind == 2 || kind == 3) && !mf)
-            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs FP32 and p = 3 or 4");
+            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs FP32 and p = 1..4");
         if (kind < 1 || kind > 3) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1, 2 or 3");
         LaunchShape s;
         s.kind = kind;
@@ -608,6 +608,44 @@ class SolverT : public Solver {
         steps = hctrl.steps;
         return Status::ok();
     }
+
+    void slot_info(SlotInfo* o) override {
+        o->c = c;
+        o->acc = acc;
+        o->ua = ua;
+        o->ub = ub;
+        o->geo_e = geo_e;
+        o->geo_f = geo_f;
+        o->nbr = nbr;
+        o->fword = fword;
+        o->halo = halo;
+        o->ctrl = ctrl;
+        o->K = K;
+        o->Kp = Kp;
+        o->N = N;
+        o->p = p;
+        o->hcap = mx.hcap;
+        o->kind = shape.kind;
+        o->phys = phys;
+    }
+
+    Status begin(double t_end, bool has_end, double cfl) override {
+        Status st = begin_run(t_end, has_end, cfl);
+        if (!st) return st;
+        HIPST(hipStreamSynchronize(stream));
+        return Status::ok();
+    }
+
+    Status refresh() override {
+        Status st = pull_ctrl();
+        if (!st) return st;
+        t = hctrl.t;
+        steps = hctrl.steps;
+        return Status::ok();
+    }
+
+    void set_lambda_valid(bool v) override { lam_valid = v; }
+    hipStream_t get_stream() const override { return stream; }
 
     Status set_kernel(int kind) override {
         Status st = choose_shape(kind);

@@ -1,0 +1,182 @@
+// One element per lane (VALU): the fused stage body shared by the
+// single-solve VALU kernel and the batched kernel.
+//
+// One call does what the reference does with four launches
+// (eval_surface, eval_volume, eval_rhs, rk_tempstorage;
+// src/2d/conservation/time_integrator.cu:274-307):
+//   volume integral    (conserv_kernels.cu:766-811)
+//   LLF surface terms  (conserv_kernels.cu:628-758) evaluated for the three
+//                      faces of the element, each face flux computed in the
+//                      side's canonical left->right orientation so both
+//                      elements of a side obtain bit-identical fluxes
+//                      (race-free, no atomics, no face buffers)
+//   gather + M^-1      (time_integrator.cu:130-188, M = J*I)
+//   RK stage update    (time_integrator.cu:194-218, 445-464)
+//   CFL wave speed     (euler.cu:252-278) of the new state on the last stage.
+// The FP64 instantiation follows the reference's operation order.
+#pragma once
+#include "dg_kernels.h"
+
+namespace dgmi {
+
+// phi on face f at point index j where j is a runtime value in [0, NQ1)
+// given as the compile-time pair (j, NQ1-1-j) selected by `rev`
+template <typename R, int P>
+__device__ __forceinline__ R face_phi(const TabP<R, P>& T, int f, int m, int j, bool rev) {
+    constexpr int NQ1 = P + 1;
+    return rev ? T.S[f][m][NQ1 - 1 - j] : T.S[f][m][j];
+}
+
+// Processes element e (clamped for loads when invalid, no stores then);
+// returns its cell wave speed after the last stage of a step, else 0.
+template <int N, int P, typename R>
+__device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int kind) {
+    constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1;
+    const TabP<R, P>& T = tab<R, P>();
+    const bool valid = e < a.K;
+    const int ee = valid ? e : a.K - 1;
+    const int Kp = a.Kp;
+    const R dt = (R)a.ctrl->dt;
+    R u[N][NP];
+#pragma unroll
+    for (int v = 0; v < N; v++)
+#pragma unroll
+        for (int m = 0; m < NP; m++) u[v][m] = a.u[(size_t)(v * NP + m) * Kp + ee];
+
+    R tot[N][NP];
+#pragma unroll
+    for (int v = 0; v < N; v++)
+#pragma unroll
+        for (int m = 0; m < NP; m++) tot[v][m] = R(0);
+
+    // ---- volume: sum_q F(U_q) . (metric * grad phi_m) ----
+    {
+        const R xr = a.geo_e[ee], yr = a.geo_e[Kp + ee], xs = a.geo_e[2 * Kp + ee], ys = a.geo_e[3 * Kp + ee];
+#pragma unroll 2
+        for (int q = 0; q < NQ; q++) {
+            R U[N];
+#pragma unroll
+            for (int v = 0; v < N; v++) U[v] = R(0);
+#pragma unroll
+            for (int m = 0; m < NP; m++)
+#pragma unroll
+                for (int v = 0; v < N; v++) U[v] += u[v][m] * T.B[m][q];
+            R fx[N], fy[N];
+            flux<N, R>(a.phys, U, fx, fy);
+#pragma unroll
+            for (int m = 0; m < NP; m++) {
+                const R bgx = T.Gx[m][q], bgy = T.Gy[m][q];
+#pragma unroll
+                for (int v = 0; v < N; v++) tot[v][m] += fx[v] * (bgx * ys - bgy * yr) + fy[v] * (-bgx * xs + bgy * xr);
+            }
+        }
+    }
+
+    // ---- the three faces, added in face order as eval_rhs adds s1, s2, s3 ----
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const int w = a.nbr[f * Kp + ee];
+        const int nb = w & NB_ELEM_MASK;
+        const int fn = (w >> 28) & 3;
+        const bool left = (w >> 30) & 1;
+        const R nx = a.geo_f[(3 * f + 0) * Kp + ee];
+        const R ny = a.geo_f[(3 * f + 1) * Kp + ee];
+        const R len = a.geo_f[(3 * f + 2) * Kp + ee];
+        R FS[NQ1][N];
+#pragma unroll
+        for (int j = 0; j < NQ1; j++) {
+            // canonical point j of the side: the left element's trace at its
+            // point j, the right element's at NQ1-1-j
+            R Uo[N], Un[N];
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                Uo[v] = R(0);
+                Un[v] = R(0);
+            }
+#pragma unroll
+            for (int m = 0; m < NP; m++) {
+                const R so = face_phi<R, P>(T, f, m, j, !left);
+                const R s0 = face_phi<R, P>(T, 0, m, j, left);
+                const R s1 = face_phi<R, P>(T, 1, m, j, left);
+                const R s2 = face_phi<R, P>(T, 2, m, j, left);
+                const R sn = (fn == 0) ? s0 : ((fn == 1) ? s1 : s2);
+#pragma unroll
+                for (int v = 0; v < N; v++) {
+                    Uo[v] += u[v][m] * so;
+                    Un[v] += a.u[(size_t)(v * NP + m) * Kp + nb] * sn;
+                }
+            }
+            R UL[N], UR[N];
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                UL[v] = left ? Uo[v] : Un[v];
+                UR[v] = left ? Un[v] : Uo[v];
+            }
+            R fxl[N], fyl[N], fxr[N], fyr[N];
+            flux<N, R>(a.phys, UL, fxl, fyl);
+            flux<N, R>(a.phys, UR, fxr, fyr);
+            const R lam = llf_speed<N, R>(a.phys, UL, UR, nx, ny);
+#pragma unroll
+            for (int v = 0; v < N; v++)
+                FS[j][v] = T.W1[j] * (R(0.5) * ((fxl[v] + fxr[v]) * nx + (fyl[v] + fyr[v]) * ny + lam * (UL[v] - UR[v])));
+        }
+        const R coef = left ? -len / R(2) : len / R(2);
+#pragma unroll
+        for (int m = 0; m < NP; m++)
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                R s = R(0);
+#pragma unroll
+                for (int j = 0; j < NQ1; j++) s += FS[j][v] * face_phi<R, P>(T, f, m, j, !left);
+                tot[v][m] += coef * s;
+            }
+    }
+
+    // ---- inverse mass (J*I) and the RK stage update ----
+    const R J = a.geo_e[4 * Kp + ee];
+    const R cj = R(1) / J * dt;
+    double lam = 0.0;
+    if (kind == ST_RHS) {
+        if (valid)
+#pragma unroll
+            for (int v = 0; v < N; v++)
+#pragma unroll
+                for (int m = 0; m < NP; m++) a.out[(size_t)(v * NP + m) * Kp + e] = cj * tot[v][m];
+    } else {
+        R c0[N];
+#pragma unroll
+        for (int v = 0; v < N; v++)
+#pragma unroll
+            for (int m = 0; m < NP; m++) {
+                const size_t ix = (size_t)(v * NP + m) * Kp + ee;
+                const R k = cj * tot[v][m];
+                const R c = (kind == ST_RK4_1 || kind == ST_RK2_1) ? u[v][m] : a.c[ix];
+                R o;
+                if (kind == ST_RK4_1) {
+                    if (valid) a.acc[ix] = k / R(6);
+                    o = c + R(0.5) * k;
+                } else if (kind == ST_RK4_2 || kind == ST_RK4_3) {
+                    if (valid) a.acc[ix] = a.acc[ix] + k / R(3);
+                    o = (kind == ST_RK4_2) ? c + R(0.5) * k : c + R(1.0) * k;
+                } else if (kind == ST_RK4_4) {
+                    o = c + (a.acc[ix] + k / R(6));
+                } else if (kind == ST_RK2_1) {
+                    o = c + R(0.5) * k;
+                } else {
+                    o = c + k;
+                }
+                if (valid) a.out[ix] = o;
+                if (m == 0) c0[v] = o;
+            }
+        if (kind == ST_RK4_4 || kind == ST_RK2_2) {
+            double U[4];
+            const double b0 = tab<double, P>().B[0][0];
+#pragma unroll
+            for (int v = 0; v < N; v++) U[v] = (double)c0[v] * b0;
+            lam = valid ? cell_speed<N>(a.phys, U) : 0.0;
+        }
+    }
+    return lam;
+}
+
+}  // namespace dgmi

@@ -1,0 +1,100 @@
+"""Batched solves: one persistent launch per RK stage over many solves of
+mixed size, order and physics.
+
+Bars: each solve's result in a mix equals its result alone (bit for bit:
+no cross-solve reductions, per-solve dt); FP32 batch equals the single-solve
+matrix-core path bit for bit; FP64 batch within 1e-12 relative L2 of the
+FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import dgcuda_amd as dg
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def mix(seed=5, n=8):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx = int(rng.integers(4, 20))
+        p = int(rng.integers(1, 5))
+        m = dg.Mesh.periodic_jittered(nx, nx, 0.0, 10.0, 0.0, 10.0, 0.1, int(rng.integers(1, 1000)))
+        if i % 2 == 0:
+            a = rng.uniform(-1, 1, 2)
+            prob = dg.advection_gaussian(a=(float(a[0]), float(a[1])), center=tuple(rng.uniform(0, 10, 2)), k=0.3)
+        else:
+            prob = dg.euler_vortex(((float(rng.uniform(2, 5)), float(rng.uniform(0, 10)), float(rng.uniform(0, 10))),))
+        out.append((m, prob, p, float(rng.uniform(0.2, 0.6))))
+    return out
+
+
+@pytest.mark.parametrize("prec", [dg.DG_FP32, dg.DG_FP64])
+def test_batch_alone_equals_in_mix(prec):
+    cases = mix()
+    b = dg.Batch(prec)
+    for m, prob, p, te in cases:
+        b.add(m, prob, p, te)
+    b.project_ic()
+    b.integrate(0.3)
+    for k in (0, 3, 6):
+        m, prob, p, te = cases[k]
+        b1 = dg.Batch(prec)
+        b1.add(m, prob, p, te)
+        b1.project_ic()
+        b1.integrate(0.3)
+        assert np.array_equal(b.get_state(k), b1.get_state(0)), k
+        assert b.audit(k)["t"] == te
+
+
+def test_batch_fp32_equals_single_mfma():
+    cases = mix(seed=9, n=6)
+    b = dg.Batch(dg.DG_FP32)
+    for m, prob, p, te in cases:
+        b.add(m, prob, p, te)
+    b.project_ic()
+    b.run_steps(12, 0.3)
+    for k, (m, prob, p, te) in enumerate(cases):
+        s = dg.Solver(m, prob, p, dg.DG_FP32)
+        s.set_kernel(2)
+        s.project_ic()
+        s.run_steps(12, 0.3)
+        got, want = b.get_state(k), s.get_state()
+        err = float(np.abs(got - want).max() / np.abs(want).max())
+        print("slot %d p=%d N=%d max rel diff %.3e" % (k, p, prob.physics, err))
+        assert err <= 1e-6
+
+
+def test_batch_fp64_vs_oracle():
+    cases = mix(seed=11, n=5)
+    b = dg.Batch(dg.DG_FP64)
+    for m, prob, p, te in cases:
+        b.add(m, prob, p, 0.15)
+    b.project_ic()
+    b.integrate(0.3)
+    for k, (m, prob, p, te) in enumerate(cases):
+        o = orc.Oracle.from_mesh(m, p, prob)
+        u0 = o.project_ic(orc.ic_array(prob, m.period))
+        ref, n, _ = o.integrate(u0, 0.15, 0.3)
+        err = orc.rel_l2(b.get_state(k), ref)
+        print("slot %d p=%d N=%d steps %d rel L2 %.3e" % (k, p, prob.physics, n, err))
+        assert err <= 1e-12
+
+
+def test_batch_fp32_mfma_low_order_vs_oracle():
+    """The matrix-core body at p = 1, 2 (used by FP32 batches)."""
+    for p in (1, 2):
+        m = dg.Mesh.periodic_jittered(12, 12, 0.0, 10.0, 0.0, 10.0, 0.1, 4)
+        prob = dg.euler_vortex(((4.0, 5.0, 5.0),))
+        o = orc.Oracle.from_mesh(m, p, prob)
+        u0 = o.project_ic(orc.ic_array(prob, m.period))
+        ref, _, _ = o.integrate(u0, 0.3, 0.3)
+        s = dg.Solver(m, prob, p, dg.DG_FP32)
+        s.set_kernel(2)
+        s.project_ic()
+        s.integrate(0.3, 0.3)
+        err = orc.rel_l2(s.get_state(), ref)
+        print("p=%d rel L2 %.3e" % (p, err))
+        assert err <= 1e-5
