@@ -31,6 +31,7 @@ DG_IC_CONSTANT, DG_IC_GAUSSIAN, DG_IC_VORTEX = 0, 1, 2
 DG_QUIRK_FLOAT_SQRT, DG_QUIRK_FLOAT_GEOM = 1, 2
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_IO", 3: "DG_ERR_PARSE", 4: "DG_ERR_HIP",
                 5: "DG_ERR_UNSUPPORTED", 6: "DG_ERR_NUMERIC", 7: "DG_ERR_BUDGET", 8: "DG_ERR_STATE"}
+STATUS_CODES = {v: k for k, v in STATUS_NAMES.items()}
 
 
 class DGError(RuntimeError):

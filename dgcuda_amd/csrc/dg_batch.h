@@ -5,6 +5,26 @@
 
 namespace dgmi {
 
+// Device side of a batch (slot table, work list, persistent launches, step
+// graph), shared by Batch and the serving loop.
+class BatchEngine {
+   public:
+    static Status create(int precision, int max_slots, int max_items, BatchEngine** out);
+    virtual ~BatchEngine() {}
+    // uploads the slot table and work list of these solves (stream-ordered)
+    virtual Status set_slots(const std::vector<Solver*>& solvers) = 0;
+    // enqueues one step (per-solve dt + 4 stages) for every resident solve
+    virtual Status step(bool use_graph) = 0;
+    // per slot: done flag, steps, t (waits for the engine's stream)
+    virtual Status status(std::vector<double>& out) = 0;
+    virtual Status sync() = 0;
+    virtual hipStream_t stream() = 0;
+    virtual int max_tile_halo() const = 0;
+};
+
+// a solver set up for batched stepping (FP32: matrix-core tiling at every p)
+Status make_batch_solver(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out);
+
 class Batch {
    public:
     static Status create(int precision, Batch** out);

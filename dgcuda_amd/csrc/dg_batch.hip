@@ -135,10 +135,11 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage(const SlotDesc<R>* __res
     }
 }
 
+// slot count on the device, so a captured step stays valid as solves come and go
 template <typename R>
-__global__ void k_batch_dt(const SlotDesc<R>* __restrict__ slots, int nslots) {
+__global__ void k_batch_dt(const SlotDesc<R>* __restrict__ slots, const int* __restrict__ nslots) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nslots) dt_update(slots[s].ctrl);
+    if (s < *nslots) dt_update(slots[s].ctrl);
 }
 
 // done flag, steps and t of every slot into one array (one host copy)
@@ -159,70 +160,88 @@ __global__ void k_batch_status(const SlotDesc<R>* __restrict__ slots, int nslots
         if (_e != hipSuccess) return hip_status(_e, #expr);  \
     } while (0)
 
+// Device side of a batch: slot table, work list, XCD partition, the
+// persistent launch shape and the captured step.  The grid and LDS size are
+// fixed at creation (LDS for the largest tile halo), and the slot count
+// lives on the device, so the step graph stays valid while set_slots
+// changes which solves are resident, as long as the capacities hold.
 template <typename R>
-class BatchT : public Batch {
+class EngineT : public BatchEngine {
    public:
-    std::vector<Solver*> solvers;
-    std::vector<double> t_end;
-    hipStream_t stream = nullptr;
+    hipStream_t st = nullptr;
     SlotDesc<R>* d_slots = nullptr;
     int2* d_items = nullptr;
-    int* d_xcd = nullptr;
+    int* d_xcd = nullptr;   // [9] XCD chunk bounds, then [1] slot count
     double* d_status = nullptr;
     int cap_slots = 0, cap_items = 0;
-    int nitems = 0, grid = 0;
+    int nslots = 0, nitems = 0, grid = 0, dt_grid = 0;
     size_t lds = 0;
-    bool dirty = true;
     hipGraphExec_t gexec = nullptr;
-    int64_t bytes = 0;
 
-    ~BatchT() override {
+    ~EngineT() override {
         if (gexec) hipGraphExecDestroy(gexec);
-        for (Solver* s : solvers) delete s;
         void* b[] = {d_slots, d_items, d_xcd, d_status};
-        for (void* p : b)
-            if (p) hipFree(p);
-        if (stream) hipStreamDestroy(stream);
+        for (void* q : b)
+            if (q) hipFree(q);
+        if (st) hipStreamDestroy(st);
     }
 
-    Status init() {
-        Status st = ensure_device_ready();
-        if (!st) return st;
-        HIPST(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    Status init(int max_slots, int max_items) {
+        Status s0 = ensure_device_ready();
+        if (!s0) return s0;
+        HIPST(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPST(hipMalloc((void**)&d_xcd, 10 * sizeof(int)));
+        HIPST(hipMemsetAsync(d_xcd, 0, 10 * sizeof(int), st));
+        lds = (sizeof(R) == 4) ? mfma_tile_lds_bytes(4, MFMA_MAX_HALO) : 0;
+        int per_cu = 0, dev = 0, cus = 0;
+        HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_batch_stage<R, ST_RK4_2>), 256, lds));
+        HIPST(hipGetDevice(&dev));
+        HIPST(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        grid = std::max(8, per_cu * cus / 8 * 8);
+        Status s1 = reserve(max_slots, max_items);
+        if (!s1) return s1;
+        HIPST(hipStreamSynchronize(st));
         return Status::ok();
     }
 
-    Status add(const Mesh& m, const dg_problem_desc& prob, int p, double te, int* slot) override {
-        if (p < 1 || p > 4) return Status::err(DG_ERR_UNSUPPORTED, "batched solves support p = 1..4");
-        if (!(te > 0)) return Status::err(DG_ERR_ARG, "end time must be positive");
-        Solver* s = nullptr;
-        Status st = Solver::create(m, prob, p, sizeof(R) == 8 ? DG_FP64 : DG_FP32, &s);
-        if (!st) return st;
-        if (sizeof(R) == 4) {
-            st = s->set_kernel(2);
-            if (!st) {
-                delete s;
-                return st;
-            }
+    // grows the device tables; invalidates the captured step when it does
+    Status reserve(int ns, int ni) {
+        if (ns > cap_slots) {
+            if (d_slots) hipFree(d_slots);
+            if (d_status) hipFree(d_status);
+            cap_slots = std::max(ns, 2 * cap_slots);
+            HIPST(hipMalloc((void**)&d_slots, cap_slots * sizeof(SlotDesc<R>)));
+            HIPST(hipMalloc((void**)&d_status, cap_slots * 3 * sizeof(double)));
+            dt_grid = (cap_slots + 63) / 64;
+            drop_graph();
         }
-        solvers.push_back(s);
-        t_end.push_back(te);
-        dirty = true;
-        if (slot) *slot = (int)solvers.size() - 1;
+        if (ni > cap_items) {
+            if (d_items) hipFree(d_items);
+            cap_items = std::max(ni, 2 * cap_items);
+            HIPST(hipMalloc((void**)&d_items, cap_items * sizeof(int2)));
+            drop_graph();
+        }
         return Status::ok();
     }
 
-    // slot table, work list, XCD partition and launch shape
-    Status rebuild() {
+    void drop_graph() {
+        if (gexec) {
+            hipGraphExecDestroy(gexec);
+            gexec = nullptr;
+        }
+    }
+
+    Status set_slots(const std::vector<Solver*>& solvers) override {
         const int ns = (int)solvers.size();
-        if (ns == 0) return Status::err(DG_ERR_STATE, "empty batch");
         std::vector<SlotDesc<R>> hs(ns);
         std::vector<int2> hi;
         std::vector<double> cost;
-        size_t max_lds = 0;
         for (int s = 0; s < ns; s++) {
             SlotInfo si;
             solvers[s]->slot_info(&si);
+            if (sizeof(R) == 4 && si.kind < 2) return Status::err(DG_ERR_STATE, "FP32 batch slot without matrix-core tiling");
+            if (si.hcap > MFMA_MAX_HALO) return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
             SlotDesc<R>& d = hs[s];
             d.c = (R*)si.c;
             d.acc = (R*)si.acc;
@@ -243,111 +262,170 @@ class BatchT : public Batch {
             d.phys = si.phys;
             const int te = (d.path == 2) ? MFMA_TILE : BATCH_VALU_TILE;
             const int nt = (si.K + te - 1) / te;
-            if (d.path == 2) max_lds = std::max(max_lds, mfma_tile_lds_bytes(si.N, si.hcap));
             const double per_elem = (double)si.N * n_modes(si.p) * sizeof(R);
             for (int t = 0; t < nt; t++) {
                 hi.push_back(int2{s, t});
                 cost.push_back(per_elem * std::min(te, si.K - t * te));
             }
         }
-        nitems = (int)hi.size();
-        // 8 contiguous chunks of equal estimated cost (one per XCD)
-        std::vector<int> xs(9, nitems);
+        const int ni = (int)hi.size();
+        // 8 contiguous chunks of equal estimated cost (one per XCD), then the slot count
+        std::vector<int> xs(10, ni);
         xs[0] = 0;
         const double total = std::accumulate(cost.begin(), cost.end(), 0.0);
         double run = 0;
         int x = 1;
-        for (int i = 0; i < nitems && x < 8; i++) {
+        for (int i = 0; i < ni && x < 8; i++) {
             run += cost[i];
             while (x < 8 && run >= total * x / 8.0) xs[x++] = i + 1;
         }
-        for (; x < 8; x++) xs[x] = nitems;
-        if (ns > cap_slots) {
-            if (d_slots) hipFree(d_slots);
-            if (d_status) hipFree(d_status);
-            cap_slots = std::max(ns, 2 * cap_slots);
-            HIPST(hipMalloc((void**)&d_slots, cap_slots * sizeof(SlotDesc<R>)));
-            HIPST(hipMalloc((void**)&d_status, cap_slots * 3 * sizeof(double)));
-        }
-        if (nitems > cap_items) {
-            if (d_items) hipFree(d_items);
-            cap_items = std::max(nitems, 2 * cap_items);
-            HIPST(hipMalloc((void**)&d_items, cap_items * sizeof(int2)));
-        }
-        if (!d_xcd) HIPST(hipMalloc((void**)&d_xcd, 9 * sizeof(int)));
-        HIPST(hipMemcpyAsync(d_slots, hs.data(), ns * sizeof(SlotDesc<R>), hipMemcpyHostToDevice, stream));
-        HIPST(hipMemcpyAsync(d_items, hi.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, stream));
-        HIPST(hipMemcpyAsync(d_xcd, xs.data(), 9 * sizeof(int), hipMemcpyHostToDevice, stream));
-        HIPST(hipStreamSynchronize(stream));
-        lds = max_lds;
-        // persistent grid: the workgroups that fit at once, a multiple of 8
-        int per_cu = 0, dev = 0, cus = 0;
-        HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&k_batch_stage<R, ST_RK4_2>), 256, lds));
-        HIPST(hipGetDevice(&dev));
-        HIPST(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid = std::max(8, std::min(per_cu * cus, (nitems + 7) / 8 * 8) / 8 * 8);
-        if (gexec) {
-            hipGraphExecDestroy(gexec);
-            gexec = nullptr;
-        }
-        dirty = false;
+        xs[9] = ns;
+        Status s0 = reserve(std::max(ns, 1), std::max(ni, 1));
+        if (!s0) return s0;
+        if (ns) HIPST(hipMemcpyAsync(d_slots, hs.data(), ns * sizeof(SlotDesc<R>), hipMemcpyHostToDevice, st));
+        if (ni) HIPST(hipMemcpyAsync(d_items, hi.data(), ni * sizeof(int2), hipMemcpyHostToDevice, st));
+        HIPST(hipMemcpyAsync(d_xcd, xs.data(), 10 * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPST(hipStreamSynchronize(st));  // the host vectors go out of scope
+        nslots = ns;
+        nitems = ni;
         return Status::ok();
     }
 
-    Status stage(int kind) {
+    Status launch_stage(int kind) {
         const dim3 g(grid), b(256);
         switch (kind) {
-            case ST_RK4_1: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_1>), g, b, lds, stream, d_slots, d_items, d_xcd); break;
-            case ST_RK4_2: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_2>), g, b, lds, stream, d_slots, d_items, d_xcd); break;
-            case ST_RK4_3: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_3>), g, b, lds, stream, d_slots, d_items, d_xcd); break;
-            case ST_RK4_4: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_4>), g, b, lds, stream, d_slots, d_items, d_xcd); break;
+            case ST_RK4_1: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_1>), g, b, lds, st, d_slots, d_items, d_xcd); break;
+            case ST_RK4_2: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_2>), g, b, lds, st, d_slots, d_items, d_xcd); break;
+            case ST_RK4_3: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_3>), g, b, lds, st, d_slots, d_items, d_xcd); break;
+            case ST_RK4_4: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_4>), g, b, lds, st, d_slots, d_items, d_xcd); break;
             default: return Status::err(DG_ERR_ARG, "unsupported batch stage");
         }
         return hip_status(hipGetLastError(), "batch stage launch");
     }
 
-    Status enqueue_step_direct() {
-        const int ns = (int)solvers.size();
-        hipLaunchKernelGGL((k_batch_dt<R>), dim3((ns + 63) / 64), dim3(64), 0, stream, d_slots, ns);
+    Status step_direct() {
+        hipLaunchKernelGGL((k_batch_dt<R>), dim3(dt_grid), dim3(64), 0, st, d_slots, d_xcd + 9);
         HIPST(hipGetLastError());
-        Status st;
+        Status s0;
         for (int k : {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4})
-            if (!(st = stage(k))) return st;
+            if (!(s0 = launch_stage(k))) return s0;
         return Status::ok();
     }
 
-    Status enqueue_step(bool use_graph) {
-        if (!use_graph) return enqueue_step_direct();
+    Status step(bool use_graph) override {
+        if (!use_graph) return step_direct();
         if (!gexec) {
             hipGraph_t gr = nullptr;
-            HIPST(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            Status st = enqueue_step_direct();
-            hipError_t ee = hipStreamEndCapture(stream, &gr);
-            if (!st) {
+            HIPST(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            Status s0 = step_direct();
+            hipError_t ee = hipStreamEndCapture(st, &gr);
+            if (!s0) {
                 if (gr) hipGraphDestroy(gr);
-                return st;
+                return s0;
             }
             HIPST(ee);
             hipError_t ei = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
             hipGraphDestroy(gr);
             HIPST(ei);
         }
-        HIPST(hipGraphLaunch(gexec, stream));
+        HIPST(hipGraphLaunch(gexec, st));
+        return Status::ok();
+    }
+
+    Status status(std::vector<double>& out) override {
+        out.assign(3 * (size_t)nslots, 0.0);
+        if (nslots == 0) return Status::ok();
+        hipLaunchKernelGGL((k_batch_status<R>), dim3((nslots + 63) / 64), dim3(64), 0, st, d_slots, nslots, d_status);
+        HIPST(hipGetLastError());
+        HIPST(hipMemcpyAsync(out.data(), d_status, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPST(hipStreamSynchronize(st));
+        return Status::ok();
+    }
+
+    Status sync() override {
+        HIPST(hipStreamSynchronize(st));
+        return Status::ok();
+    }
+    hipStream_t stream() override { return st; }
+    int max_tile_halo() const override { return MFMA_MAX_HALO; }
+};
+
+Status BatchEngine::create(int precision, int max_slots, int max_items, BatchEngine** out) {
+    Status s0;
+    if (precision == DG_FP32) {
+        auto* e = new EngineT<float>();
+        if (!(s0 = e->init(max_slots, max_items))) {
+            delete e;
+            return s0;
+        }
+        *out = e;
+    } else if (precision == DG_FP64) {
+        auto* e = new EngineT<double>();
+        if (!(s0 = e->init(max_slots, max_items))) {
+            delete e;
+            return s0;
+        }
+        *out = e;
+    } else {
+        return Status::err(DG_ERR_ARG, "precision must be DG_FP32 or DG_FP64");
+    }
+    return Status::ok();
+}
+
+Status make_batch_solver(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out) {
+    if (p < 1 || p > 4) return Status::err(DG_ERR_UNSUPPORTED, "batched solves support p = 1..4");
+    Solver* s = nullptr;
+    Status st = Solver::create(m, prob, p, precision, &s);
+    if (!st) return st;
+    if (precision == DG_FP32 && !(st = s->set_kernel(2))) {
+        delete s;
+        return st;
+    }
+    *out = s;
+    return Status::ok();
+}
+
+// The user-facing batch: solves added up front, run together.
+class BatchImpl : public Batch {
+   public:
+    int precision = DG_FP32;
+    std::vector<Solver*> solvers;
+    std::vector<double> t_end;
+    BatchEngine* eng = nullptr;
+    bool dirty = true;
+
+    ~BatchImpl() override {
+        delete eng;
+        for (Solver* s : solvers) delete s;
+    }
+
+    Status add(const Mesh& m, const dg_problem_desc& prob, int p, double te, int* slot) override {
+        if (!(te > 0)) return Status::err(DG_ERR_ARG, "end time must be positive");
+        Solver* s = nullptr;
+        Status st = make_batch_solver(m, prob, p, precision, &s);
+        if (!st) return st;
+        solvers.push_back(s);
+        t_end.push_back(te);
+        dirty = true;
+        if (slot) *slot = (int)solvers.size() - 1;
         return Status::ok();
     }
 
     Status begin_all(bool has_end, double cfl) {
+        if (solvers.empty()) return Status::err(DG_ERR_STATE, "empty batch");
         Status st;
-        if (dirty && !(st = rebuild())) return st;
+        if (dirty) {
+            if (!(st = eng->set_slots(solvers))) return st;
+            dirty = false;
+        }
         for (size_t s = 0; s < solvers.size(); s++)
             if (!(st = solvers[s]->begin(t_end[s], has_end, cfl))) return st;
         return Status::ok();
     }
 
     Status finish_all() {
-        HIPST(hipStreamSynchronize(stream));
-        Status st;
+        Status st = eng->sync();
+        if (!st) return st;
         for (Solver* s : solvers) {
             if (!(st = s->refresh())) return st;
             s->set_lambda_valid(true);
@@ -367,7 +445,7 @@ class BatchT : public Batch {
         Status st = begin_all(false, cfl);
         if (!st) return st;
         for (int i = 0; i < nsteps; i++)
-            if (!(st = enqueue_step(use_graph))) return st;
+            if (!(st = eng->step(use_graph))) return st;
         return finish_all();
     }
 
@@ -376,17 +454,14 @@ class BatchT : public Batch {
         Status st = begin_all(true, cfl);
         if (!st) return st;
         const int ns = (int)solvers.size();
-        std::vector<double> h(3 * ns);
+        std::vector<double> h;
         int64_t issued = 0;
         int chunk = 8;
         while (true) {
             for (int i = 0; i < chunk; i++)
-                if (!(st = enqueue_step(use_graph))) return st;
+                if (!(st = eng->step(use_graph))) return st;
             issued += chunk;
-            hipLaunchKernelGGL((k_batch_status<R>), dim3((ns + 63) / 64), dim3(64), 0, stream, d_slots, ns, d_status);
-            HIPST(hipGetLastError());
-            HIPST(hipMemcpyAsync(h.data(), d_status, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-            HIPST(hipStreamSynchronize(stream));
+            if (!(st = eng->status(h))) return st;
             bool all = true;
             for (int s = 0; s < ns; s++) all = all && (h[3 * s] != 0.0 || !(h[3 * s + 2] < t_end[s]));
             if (all) break;
@@ -398,13 +473,15 @@ class BatchT : public Batch {
 
     Status get_state(int slot, double* c_ref) override {
         if (slot < 0 || slot >= (int)solvers.size()) return Status::err(DG_ERR_ARG, "slot out of range");
-        HIPST(hipStreamSynchronize(stream));
+        Status st = eng->sync();
+        if (!st) return st;
         return solvers[slot]->get_state(c_ref);
     }
 
     Status audit(int slot, dg_audit_out* out) override {
         if (slot < 0 || slot >= (int)solvers.size()) return Status::err(DG_ERR_ARG, "slot out of range");
-        HIPST(hipStreamSynchronize(stream));
+        Status st = eng->sync();
+        if (!st) return st;
         return solvers[slot]->audit(out);
     }
 
@@ -422,22 +499,14 @@ class BatchT : public Batch {
 
 Status Batch::create(int precision, Batch** out) {
     if (precision != DG_FP32 && precision != DG_FP64) return Status::err(DG_ERR_ARG, "precision must be DG_FP32 or DG_FP64");
-    Status st;
-    if (precision == DG_FP32) {
-        auto* b = new BatchT<float>();
-        if (!(st = b->init())) {
-            delete b;
-            return st;
-        }
-        *out = b;
-    } else {
-        auto* b = new BatchT<double>();
-        if (!(st = b->init())) {
-            delete b;
-            return st;
-        }
-        *out = b;
+    auto* b = new BatchImpl();
+    b->precision = precision;
+    Status st = BatchEngine::create(precision, 64, 4096, &b->eng);
+    if (!st) {
+        delete b;
+        return st;
     }
+    *out = b;
     return Status::ok();
 }
 

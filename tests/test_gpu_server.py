@@ -1,0 +1,84 @@
+"""Serving loop: continuous batching with admission by slots and HBM budget.
+
+Bars: every request's final state in the served mix is bit-identical to the
+same solve run alone; budget-limited admission completes every request; a
+request whose state goes non-physical fails alone (DG_ERR_NUMERIC) and the
+others are unaffected; mass is conserved to round-off per request.
+"""
+import numpy as np
+import pytest
+
+import dgcuda_amd as dg
+
+pytestmark = pytest.mark.gpu
+
+
+def requests(n, seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    for i in range(n):
+        t += float(rng.exponential(0.01))
+        nx = int(rng.integers(6, 24))
+        p = int(rng.integers(1, 4))
+        prob = dg.euler_vortex(((float(rng.uniform(2, 5)), float(rng.uniform(0, 10)), float(rng.uniform(0, 10))),))
+        out.append(dg.request(nx, p, float(rng.uniform(0.1, 0.4)), prob, mesh_seed=int(rng.integers(1, 10**6)),
+                              arrival_s=t))
+    return out
+
+
+def alone(req):
+    m = dg.Mesh.periodic_jittered(req.nx, req.ny, req.x0, req.x1, req.y0, req.y1, req.jitter, req.mesh_seed)
+    b = dg.Batch(dg.DG_FP32)
+    b.add(m, req.prob, req.p, req.t_end)
+    b.project_ic()
+    b.integrate(0.3)
+    return b.get_state(0)
+
+
+def test_server_results_equal_alone():
+    reqs = requests(10)
+    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3)
+    sv.keep_states(True)
+    t0 = sv.now()
+    ids = []
+    for r in reqs:
+        r.arrival_s += t0
+        ids.append(sv.submit(r))
+    sv.drain()
+    res = {r.id: r for r in sv.poll(64)}
+    assert len(res) == len(reqs)
+    for k in (0, 4, 9):
+        r = reqs[k]
+        out = res[ids[k]]
+        assert out.status == 0 and abs(out.t - r.t_end) == 0.0
+        shape = (4, dg.n_modes(r.p), 2 * r.nx * r.ny)
+        got = sv.get_state(ids[k], shape)
+        assert np.array_equal(got, alone(r)), k
+    for out in res.values():
+        assert out.finish_s >= out.arrival_s
+        m0, m1 = np.array(out.mass0), np.array(out.mass)
+        assert np.all(np.abs(m1 - m0) <= 2e-6 * np.abs(m0) + 1e-9)
+
+
+def test_server_budget_admission():
+    reqs = requests(8, seed=4)
+    one = 40 * 2 * 24 * 24 * 4 * 15 * 4 * 2  # room for about two of the largest at a time
+    sv = dg.Server(dg.DG_FP32, max_slots=64, hbm_budget=one, steps_per_round=4, cfl=0.3)
+    for r in reqs:
+        sv.submit(r)
+    sv.drain()
+    res = sv.poll(64)
+    assert len(res) == len(reqs) and all(r.status == 0 for r in res)
+
+
+def test_server_bad_request_fails_alone():
+    reqs = requests(5, seed=6)
+    bad = dg.request(10, 2, 0.2, dg.euler_vortex(((25.0, 5.0, 5.0),)), mesh_seed=1)  # T < 0 at the core
+    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3)
+    good_ids = [sv.submit(r) for r in reqs]
+    bad_id = sv.submit(bad)
+    sv.drain()
+    res = {r.id: r for r in sv.poll(64)}
+    assert res[bad_id].status == dg.STATUS_CODES["DG_ERR_NUMERIC"]
+    assert all(res[i].status == 0 for i in good_ids)
