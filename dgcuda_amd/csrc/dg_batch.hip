@@ -37,96 +37,131 @@ struct SlotDesc {
 
 constexpr int BATCH_VALU_TILE = 256;
 
-template <int N, int P, typename R, int KIND>
-__device__ __forceinline__ double batch_valu(const StageArgs<R>& a, int e) {
-    return valu_process<N, P, R>(a, e, KIND);
+// Values read from the slot table or the work list are the same in every
+// lane, but those reads follow the kernel's own stores, so the compiler
+// cannot issue them as scalar loads and would keep pointers, sizes and
+// addresses in vector registers.  readfirstlane puts them in SGPRs.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uni(double x) {
+    const unsigned long long v = __double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T>
+__device__ __forceinline__ T* uni(T* ptr) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(ptr);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
 }
 
+// One wavefront-group's run of consecutive items of the same solve
+// (items it, it+G8, ... while they belong to `slot`), with the next tile's
+// state in flight while the current one computes, as in the single-solve
+// kernel.  Returns the first item index not processed.
 template <int N, int P, int KIND>
-__device__ __forceinline__ double batch_mfma(const StageArgs<float>& a, const MfmaArgs& mx, int tile, float4* fd,
-                                             float* ts) {
+__device__ __forceinline__ int batch_mfma_run(const StageArgs<float>& a, const MfmaArgs& mx, int slot,
+                                              const int2* __restrict__ items, int it, int i1, int G8, float4* fd,
+                                              float* ts, double& lam) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
     MfmaA<N, P> A;
-    A.init(g, j);
-    MfmaTileState<N, P> st;
-    st.load(a.u, a.K, a.Kp, mx.halo, mx.hcap, tile, w, g, j);
-    return mfma_process_tile<N, P, KIND>(a, mx, A, st, tile, fd, ts);
+    A.init(g, j, it >> 31);
+    MfmaTileState<N, P> cur, nxt;
+    int tile = uni(items[it].y);
+    cur.load(a.u, a.K, a.Kp, mx.halo, mx.hcap, tile, w, g, j);
+    while (true) {
+        const int nit = it + G8;
+        int ntile = -1;
+        if (nit < i1) {
+            const int2 ni = items[nit];
+            if (uni(ni.x) == slot) ntile = uni(ni.y);
+        }
+        if (ntile >= 0) nxt.load(a.u, a.K, a.Kp, mx.halo, mx.hcap, ntile, w, g, j);
+        lam = fmax(lam, mfma_process_tile<N, P, KIND>(a, mx, A, cur, tile, fd, ts));
+        if (ntile < 0) return nit;
+        it = nit;
+        tile = ntile;
+        cur = nxt;
+    }
 }
 
 template <typename R, int KIND>
-__global__ void __launch_bounds__(256, 2) k_batch_stage(const SlotDesc<R>* __restrict__ slots,
-                                                    const int2* __restrict__ items, const int* __restrict__ xcd_start) {
+__device__ __forceinline__ StageArgs<R> slot_args(const SlotDesc<R>* sd) {
+    StageArgs<R> a;
+    Ctrl* ctrl = uni(sd->ctrl);
+    a.ctrl = ctrl;
+    a.lam_bits = &ctrl->lam_bits;
+    a.geo_e = uni(sd->geo_e);
+    a.geo_f = uni(sd->geo_f);
+    a.nbr = uni(sd->nbr);
+    a.K = uni(sd->K);
+    a.Kp = uni(sd->Kp);
+    a.kind = KIND;
+    a.ntiles = 0;
+    a.xcd_remap = 0;
+    a.pad = 0;
+    a.phys.gamma = uni(sd->phys.gamma);
+    a.phys.ax = uni(sd->phys.ax);
+    a.phys.ay = uni(sd->phys.ay);
+    a.phys.quirks = uni(sd->phys.quirks);
+    a.phys.pad = 0;
+    R* c = uni(sd->c);
+    R* ua = uni(sd->ua);
+    R* ub = uni(sd->ub);
+    a.acc = uni(sd->acc);
+    a.c = c;
+    if constexpr (KIND == ST_RK4_1 || KIND == ST_RK2_1) {
+        a.u = c;
+        a.out = ua;
+    } else if constexpr (KIND == ST_RK4_2) {
+        a.u = ua;
+        a.out = ub;
+    } else if constexpr (KIND == ST_RK4_3) {
+        a.u = ub;
+        a.out = ua;
+    } else if constexpr (KIND == ST_RK4_4 || KIND == ST_RK2_2) {
+        a.u = ua;
+        a.out = c;
+    } else {
+        a.u = ub;
+        a.out = ua;
+    }
+    return a;
+}
+
+// Work-list layout in `parts` (XCD x = blockIdx % 8):
+//   parts[x] .. parts[x+1]          lane items of XCD x   (one element per lane)
+//   parts[9+x] .. parts[9+x+1]      tile items of XCD x   (matrix-core tiles)
+//   parts[18]                       resident slot count
+constexpr int BATCH_TILE_PART = 9;
+constexpr int BATCH_NSLOT_AT = 18;
+constexpr int BATCH_PARTS = 19;
+
+// Lane items: 256 elements each, one per thread.
+template <typename R, int KIND>
+__device__ __forceinline__ void batch_lane_items(const SlotDesc<R>* __restrict__ slots,
+                                                 const int2* __restrict__ items, int it, int i1, int G8,
+                                                 double* sh) {
     constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
-    extern __shared__ float4 lds4[];
-    __shared__ double sh[8];
-    const int x = blockIdx.x & 7, G8 = gridDim.x >> 3;
-    const int i1 = xcd_start[x + 1];
-    for (int it = xcd_start[x] + (blockIdx.x >> 3); it < i1; it += G8) {
+    for (; it < i1; it += G8) {
         const int2 item = items[it];
-        const SlotDesc<R>* sd = slots + item.x;
-        Ctrl* ctrl = sd->ctrl;
-        if (!ctrl->active) continue;
-        StageArgs<R> a;
-        a.ctrl = ctrl;
-        a.lam_bits = &ctrl->lam_bits;
-        a.geo_e = sd->geo_e;
-        a.geo_f = sd->geo_f;
-        a.nbr = sd->nbr;
-        a.K = sd->K;
-        a.Kp = sd->Kp;
-        a.kind = KIND;
-        a.ntiles = 0;
-        a.xcd_remap = 0;
-        a.pad = 0;
-        a.phys = sd->phys;
-        a.acc = sd->acc;
-        a.c = sd->c;
-        if constexpr (KIND == ST_RK4_1 || KIND == ST_RK2_1) {
-            a.u = sd->c;
-            a.out = sd->ua;
-        } else if constexpr (KIND == ST_RK4_2) {
-            a.u = sd->ua;
-            a.out = sd->ub;
-        } else if constexpr (KIND == ST_RK4_3) {
-            a.u = sd->ub;
-            a.out = sd->ua;
-        } else if constexpr (KIND == ST_RK4_4 || KIND == ST_RK2_2) {
-            a.u = sd->ua;
-            a.out = sd->c;
-        } else {
-            a.u = sd->ub;
-            a.out = sd->ua;
-        }
-        const int N = sd->N, p = sd->p;
+        const SlotDesc<R>* sd = slots + uni(item.x);
+        if (!uni(uni(sd->ctrl)->active)) continue;
+        const StageArgs<R> a = slot_args<R, KIND>(sd);
+        const int N = uni(sd->N), p = uni(sd->p);
+        const int e = uni(item.y) * BATCH_VALU_TILE + threadIdx.x;
         double lam = 0.0;
-        if constexpr (sizeof(R) == 4) {
-            // FP32: every order on the matrix-core tile body
-            MfmaArgs mx;
-            mx.fword = sd->fword;
-            mx.halo = sd->halo;
-            mx.hcap = sd->hcap;
-            mx.pad = 0;
-            float4* fd = lds4;
-            float* ts = reinterpret_cast<float*>(lds4 + MFMA_TILE * 3);
-            if (N == 4 && p == 4) lam = batch_mfma<4, 4, KIND>(a, mx, item.y, fd, ts);
-            else if (N == 4 && p == 3) lam = batch_mfma<4, 3, KIND>(a, mx, item.y, fd, ts);
-            else if (N == 4 && p == 2) lam = batch_mfma<4, 2, KIND>(a, mx, item.y, fd, ts);
-            else if (N == 4 && p == 1) lam = batch_mfma<4, 1, KIND>(a, mx, item.y, fd, ts);
-            else if (N == 1 && p == 4) lam = batch_mfma<1, 4, KIND>(a, mx, item.y, fd, ts);
-            else if (N == 1 && p == 3) lam = batch_mfma<1, 3, KIND>(a, mx, item.y, fd, ts);
-            else if (N == 1 && p == 2) lam = batch_mfma<1, 2, KIND>(a, mx, item.y, fd, ts);
-            else if (N == 1 && p == 1) lam = batch_mfma<1, 1, KIND>(a, mx, item.y, fd, ts);
-        } else {
-            const int e = item.y * BATCH_VALU_TILE + threadIdx.x;
-            if (N == 4 && p == 4) lam = batch_valu<4, 4, double, KIND>(a, e);
-            else if (N == 4 && p == 3) lam = batch_valu<4, 3, double, KIND>(a, e);
-            else if (N == 4 && p == 2) lam = batch_valu<4, 2, double, KIND>(a, e);
-            else if (N == 4 && p == 1) lam = batch_valu<4, 1, double, KIND>(a, e);
-            else if (N == 1 && p == 4) lam = batch_valu<1, 4, double, KIND>(a, e);
-            else if (N == 1 && p == 3) lam = batch_valu<1, 3, double, KIND>(a, e);
-            else if (N == 1 && p == 2) lam = batch_valu<1, 2, double, KIND>(a, e);
-            else if (N == 1 && p == 1) lam = batch_valu<1, 1, double, KIND>(a, e);
+        if constexpr (sizeof(R) == 8) {
+            if (N == 4 && p == 4) lam = valu_process<4, 4, R>(a, e, KIND);
+            else if (N == 4 && p == 3) lam = valu_process<4, 3, R>(a, e, KIND);
+            else if (N == 1 && p == 4) lam = valu_process<1, 4, R>(a, e, KIND);
+            else if (N == 1 && p == 3) lam = valu_process<1, 3, R>(a, e, KIND);
         }
+        if (N == 4 && p == 2) lam = valu_process<4, 2, R>(a, e, KIND);
+        else if (N == 4 && p == 1) lam = valu_process<4, 1, R>(a, e, KIND);
+        else if (N == 1 && p == 2) lam = valu_process<1, 2, R>(a, e, KIND);
+        else if (N == 1 && p == 1) lam = valu_process<1, 1, R>(a, e, KIND);
         if constexpr (LAST) {
             const double m = block_max(lam, sh);
             if (threadIdx.x == 0) lam_commit(m, a.lam_bits);
@@ -135,7 +170,81 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage(const SlotDesc<R>* __res
     }
 }
 
-// slot count on the device, so a captured step stays valid as solves come and go
+// One persistent launch per stage.  Each workgroup first takes its share of
+// its XCD's lane items, then of its tile items; the two walks are separate
+// loops so the register allocation of one body does not burden the other.
+// FP32 solves use lane items for p <= 2 and tiles for p >= 3 (the kernel
+// choice of a single solve, so results match it bit for bit); FP64 solves
+// use lane items at every order.
+template <int KIND>
+__global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float>* __restrict__ slots,
+                                                           const int2* __restrict__ items,
+                                                           const int* __restrict__ parts) {
+    constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
+    extern __shared__ float4 lds4[];
+    __shared__ double sh[8];
+    float4* fd = lds4;
+    float* ts = reinterpret_cast<float*>(lds4 + MFMA_TILE * 3);
+    const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
+    batch_lane_items<float, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
+
+    const int i1 = parts[BATCH_TILE_PART + x + 1];
+    int it = parts[BATCH_TILE_PART + x] + bi;
+    while (it < i1) {
+        const int slot = uni(items[it].x);
+        const SlotDesc<float>* sd = slots + slot;
+        if (!uni(uni(sd->ctrl)->active)) {
+            it += G8;
+            continue;
+        }
+        const StageArgs<float> a = slot_args<float, KIND>(sd);
+        MfmaArgs mx;
+        mx.fword = uni(sd->fword);
+        mx.halo = uni(sd->halo);
+        mx.hcap = uni(sd->hcap);
+        mx.pad = 0;
+        const int N = uni(sd->N), p = uni(sd->p);
+        double lam = 0.0;
+        int next = it + G8;
+        if (N == 4 && p == 4) next = batch_mfma_run<4, 4, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, lam);
+        else if (N == 4 && p == 3) next = batch_mfma_run<4, 3, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, lam);
+        else if (N == 1 && p == 4) next = batch_mfma_run<1, 4, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, lam);
+        else if (N == 1 && p == 3) next = batch_mfma_run<1, 3, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, lam);
+        it = next;
+        if constexpr (LAST) {
+            const double m = block_max(lam, sh);
+            if (threadIdx.x == 0) lam_commit(m, a.lam_bits);
+            __syncthreads();  // sh is reused by the next run
+        }
+    }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(256, 2) k_batch_stage_f64(const SlotDesc<double>* __restrict__ slots,
+                                                           const int2* __restrict__ items,
+                                                           const int* __restrict__ parts) {
+    __shared__ double sh[8];
+    const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
+    batch_lane_items<double, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
+}
+
+template <typename R, int KIND>
+struct BatchKernel;
+template <int KIND>
+struct BatchKernel<float, KIND> {
+    static const void* fn() { return reinterpret_cast<const void*>(&k_batch_stage_f32<KIND>); }
+    static void launch(dim3 g, size_t lds, hipStream_t st, const SlotDesc<float>* s, const int2* i, const int* x) {
+        hipLaunchKernelGGL((k_batch_stage_f32<KIND>), g, dim3(256), lds, st, s, i, x);
+    }
+};
+template <int KIND>
+struct BatchKernel<double, KIND> {
+    static const void* fn() { return reinterpret_cast<const void*>(&k_batch_stage_f64<KIND>); }
+    static void launch(dim3 g, size_t lds, hipStream_t st, const SlotDesc<double>* s, const int2* i, const int* x) {
+        hipLaunchKernelGGL((k_batch_stage_f64<KIND>), g, dim3(256), lds, st, s, i, x);
+    }
+};
+
 template <typename R>
 __global__ void k_batch_dt(const SlotDesc<R>* __restrict__ slots, const int* __restrict__ nslots) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,7 +280,7 @@ class EngineT : public BatchEngine {
     hipStream_t st = nullptr;
     SlotDesc<R>* d_slots = nullptr;
     int2* d_items = nullptr;
-    int* d_xcd = nullptr;   // [9] XCD chunk bounds, then [1] slot count
+    int* d_xcd = nullptr;   // BATCH_PARTS ints: lane and tile XCD chunk bounds, slot count
     double* d_status = nullptr;
     int cap_slots = 0, cap_items = 0;
     int nslots = 0, nitems = 0, grid = 0, dt_grid = 0;
@@ -190,12 +299,11 @@ class EngineT : public BatchEngine {
         Status s0 = ensure_device_ready();
         if (!s0) return s0;
         HIPST(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        HIPST(hipMalloc((void**)&d_xcd, 10 * sizeof(int)));
-        HIPST(hipMemsetAsync(d_xcd, 0, 10 * sizeof(int), st));
+        HIPST(hipMalloc((void**)&d_xcd, BATCH_PARTS * sizeof(int)));
+        HIPST(hipMemsetAsync(d_xcd, 0, BATCH_PARTS * sizeof(int), st));
         lds = (sizeof(R) == 4) ? mfma_tile_lds_bytes(4, MFMA_MAX_HALO) : 0;
         int per_cu = 0, dev = 0, cus = 0;
-        HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&k_batch_stage<R, ST_RK4_2>), 256, lds));
+        HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BatchKernel<R, ST_RK4_2>::fn(), 256, lds));
         HIPST(hipGetDevice(&dev));
         HIPST(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         grid = std::max(8, per_cu * cus / 8 * 8);
@@ -235,13 +343,15 @@ class EngineT : public BatchEngine {
     Status set_slots(const std::vector<Solver*>& solvers) override {
         const int ns = (int)solvers.size();
         std::vector<SlotDesc<R>> hs(ns);
-        std::vector<int2> hi;
-        std::vector<double> cost;
+        std::vector<int2> lane, tile;
+        std::vector<double> lane_cost, tile_cost;
         for (int s = 0; s < ns; s++) {
             SlotInfo si;
             solvers[s]->slot_info(&si);
-            if (sizeof(R) == 4 && si.kind < 2) return Status::err(DG_ERR_STATE, "FP32 batch slot without matrix-core tiling");
-            if (si.hcap > MFMA_MAX_HALO) return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
+            const bool tiles = sizeof(R) == 4 && si.kind >= 2;
+            if (sizeof(R) == 4 && (tiles != (si.p >= 3)))
+                return Status::err(DG_ERR_STATE, "FP32 batch slots use matrix-core tiles exactly when p >= 3");
+            if (tiles && si.hcap > MFMA_MAX_HALO) return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
             SlotDesc<R>& d = hs[s];
             d.c = (R*)si.c;
             d.acc = (R*)si.acc;
@@ -258,33 +368,43 @@ class EngineT : public BatchEngine {
             d.N = si.N;
             d.p = si.p;
             d.hcap = si.hcap;
-            d.path = (sizeof(R) == 4) ? 2 : 1;
+            d.path = tiles ? 2 : 1;
             d.phys = si.phys;
-            const int te = (d.path == 2) ? MFMA_TILE : BATCH_VALU_TILE;
+            const int te = tiles ? MFMA_TILE : BATCH_VALU_TILE;
             const int nt = (si.K + te - 1) / te;
             const double per_elem = (double)si.N * n_modes(si.p) * sizeof(R);
+            std::vector<int2>& list = tiles ? tile : lane;
+            std::vector<double>& cost = tiles ? tile_cost : lane_cost;
             for (int t = 0; t < nt; t++) {
-                hi.push_back(int2{s, t});
+                list.push_back(int2{s, t});
                 cost.push_back(per_elem * std::min(te, si.K - t * te));
             }
         }
+        // each list in 8 contiguous chunks of equal estimated cost, one per XCD
+        std::vector<int> xs(BATCH_PARTS, 0);
+        auto split = [&](const std::vector<double>& cost, int base, int off) {
+            const int n = (int)cost.size();
+            for (int k = 0; k <= 8; k++) xs[off + k] = base + n;
+            xs[off] = base;
+            const double total = std::accumulate(cost.begin(), cost.end(), 0.0);
+            double run = 0;
+            int x = 1;
+            for (int i = 0; i < n && x < 8; i++) {
+                run += cost[i];
+                while (x < 8 && run >= total * x / 8.0) xs[off + x++] = base + i + 1;
+            }
+        };
+        split(lane_cost, 0, 0);
+        split(tile_cost, (int)lane.size(), BATCH_TILE_PART);
+        xs[BATCH_NSLOT_AT] = ns;
+        std::vector<int2> hi(lane);
+        hi.insert(hi.end(), tile.begin(), tile.end());
         const int ni = (int)hi.size();
-        // 8 contiguous chunks of equal estimated cost (one per XCD), then the slot count
-        std::vector<int> xs(10, ni);
-        xs[0] = 0;
-        const double total = std::accumulate(cost.begin(), cost.end(), 0.0);
-        double run = 0;
-        int x = 1;
-        for (int i = 0; i < ni && x < 8; i++) {
-            run += cost[i];
-            while (x < 8 && run >= total * x / 8.0) xs[x++] = i + 1;
-        }
-        xs[9] = ns;
         Status s0 = reserve(std::max(ns, 1), std::max(ni, 1));
         if (!s0) return s0;
         if (ns) HIPST(hipMemcpyAsync(d_slots, hs.data(), ns * sizeof(SlotDesc<R>), hipMemcpyHostToDevice, st));
         if (ni) HIPST(hipMemcpyAsync(d_items, hi.data(), ni * sizeof(int2), hipMemcpyHostToDevice, st));
-        HIPST(hipMemcpyAsync(d_xcd, xs.data(), 10 * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPST(hipMemcpyAsync(d_xcd, xs.data(), BATCH_PARTS * sizeof(int), hipMemcpyHostToDevice, st));
         HIPST(hipStreamSynchronize(st));  // the host vectors go out of scope
         nslots = ns;
         nitems = ni;
@@ -292,19 +412,19 @@ class EngineT : public BatchEngine {
     }
 
     Status launch_stage(int kind) {
-        const dim3 g(grid), b(256);
+        const dim3 g(grid);
         switch (kind) {
-            case ST_RK4_1: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_1>), g, b, lds, st, d_slots, d_items, d_xcd); break;
-            case ST_RK4_2: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_2>), g, b, lds, st, d_slots, d_items, d_xcd); break;
-            case ST_RK4_3: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_3>), g, b, lds, st, d_slots, d_items, d_xcd); break;
-            case ST_RK4_4: hipLaunchKernelGGL((k_batch_stage<R, ST_RK4_4>), g, b, lds, st, d_slots, d_items, d_xcd); break;
+            case ST_RK4_1: BatchKernel<R, ST_RK4_1>::launch(g, lds, st, d_slots, d_items, d_xcd); break;
+            case ST_RK4_2: BatchKernel<R, ST_RK4_2>::launch(g, lds, st, d_slots, d_items, d_xcd); break;
+            case ST_RK4_3: BatchKernel<R, ST_RK4_3>::launch(g, lds, st, d_slots, d_items, d_xcd); break;
+            case ST_RK4_4: BatchKernel<R, ST_RK4_4>::launch(g, lds, st, d_slots, d_items, d_xcd); break;
             default: return Status::err(DG_ERR_ARG, "unsupported batch stage");
         }
         return hip_status(hipGetLastError(), "batch stage launch");
     }
 
     Status step_direct() {
-        hipLaunchKernelGGL((k_batch_dt<R>), dim3(dt_grid), dim3(64), 0, st, d_slots, d_xcd + 9);
+        hipLaunchKernelGGL((k_batch_dt<R>), dim3(dt_grid), dim3(64), 0, st, d_slots, d_xcd + BATCH_NSLOT_AT);
         HIPST(hipGetLastError());
         Status s0;
         for (int k : {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4})
@@ -377,7 +497,7 @@ Status make_batch_solver(const Mesh& m, const dg_problem_desc& prob, int p, int 
     Solver* s = nullptr;
     Status st = Solver::create(m, prob, p, precision, &s);
     if (!st) return st;
-    if (precision == DG_FP32 && !(st = s->set_kernel(2))) {
+    if (precision == DG_FP32 && !(st = s->set_kernel(p >= 3 ? 2 : 1))) {
         delete s;
         return st;
     }

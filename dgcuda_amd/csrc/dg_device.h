@@ -183,6 +183,16 @@ __device__ __forceinline__ int tile_of_block(int b, int nblocks, int remap) {
 // reference's euler.cu:61-168 (pressure, eval_c, eval_flux, eval_lambda) and
 // of the advection flux F = (ax u, ay u) with the LLF speed |a.n|.
 // ---------------------------------------------------------------------------
+// Pointers that arrive through memory (the batch slot table) are generic
+// to the compiler, which then issues FLAT accesses with 64-bit per-lane
+// addresses.  Reading and writing through a global-address-space view
+// gives scalar-base global loads whatever the pointer's origin.
+#define DG_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ DG_GLOBAL T* gp(T* p) {
+    return (DG_GLOBAL T*)p;
+}
+
 template <typename R>
 __device__ __forceinline__ R rdiv(R a, R b);
 template <>

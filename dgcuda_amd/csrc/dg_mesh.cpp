@@ -104,13 +104,20 @@ std::string make_periodic_jittered(int nx, int ny, double x0, double x1, double 
                 vy[vid(i, j)] += (2.0 * uy - 1.0) * jitter * hy;
             }
         }
-    // cells in Morton order
+    // cells in blocks of about 8 x 4 (64 triangles, one tile of the
+    // matrix-core kernel, so tiles stay compact whatever nx, ny are): the
+    // block columns / rows partition the grid evenly (sizes differ by at most
+    // one cell), blocks in Morton order, cells row-major inside a block
     const int C = nx * ny;
+    const int nbx = (nx + 7) / 8, nby = (ny + 3) / 4;
     std::vector<int> cells(C);
     std::iota(cells.begin(), cells.end(), 0);
-    std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) {
-        return morton2(a % nx, a / nx) < morton2(b % nx, b / nx);
-    });
+    auto key = [&](int c) {
+        const int i = c % nx, j = c / nx;
+        const int bi = (int)((int64_t)i * nbx / nx), bj = (int)((int64_t)j * nby / ny);
+        return ((uint64_t)morton2(bi, bj) << 32) | ((uint64_t)j << 16) | (uint64_t)i;
+    };
+    std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) { return key(a) < key(b); });
     std::vector<int> cell_rank(C);
     for (int r = 0; r < C; r++) cell_rank[cells[r]] = r;
     const int K = 2 * C;

@@ -36,8 +36,10 @@ std::string write_pmsh(const char* path, const Mesh& m);
 // independent uniform offsets in [-jitter*h, jitter*h] per coordinate, drawn
 // from splitmix64(seed) in row-major vertex order (two draws per vertex, x
 // then y, for every vertex 0 <= i < n_x, 0 <= j < n_y; draws for boundary
-// vertices are made and discarded).  Cells are enumerated in Morton order of
-// (i, j); cell c owns elements 2c (lower) and 2c+1 (upper).  Sides are
+// vertices are made and discarded).  Cells are enumerated in blocks of about
+// 8 x 4 cells (ceil(nx/8) x ceil(ny/4) blocks partitioning the grid evenly)
+// taken in Morton order of the block index, row-major inside a block;
+// the c-th cell owns elements 2c (lower) and 2c+1 (upper).  Sides are
 // numbered in order of first appearance over (element, local face), and the
 // element that first meets a side is its left element.
 std::string make_periodic_jittered(int nx, int ny, double x0, double x1, double y0, double y1, double jitter,

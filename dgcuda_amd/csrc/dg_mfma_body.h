@@ -51,7 +51,7 @@ __device__ __forceinline__ void load_state(const float* __restrict__ u, int Kp, 
     for (int v = 0; v < N; v++)
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            const float* bp = u + (size_t)(v * NP + 4 * s) * Kp;  // uniform
+            const DG_GLOBAL float* bp = gp(u) + (size_t)(v * NP + 4 * s) * Kp;  // uniform
             dst[v][s] = (4 * s + g < NP) ? bp[lane_off] : 0.f;   // lane_off = g*Kp + e
         }
 }
@@ -63,7 +63,7 @@ __device__ __forceinline__ void load_state_var(const float* __restrict__ u, int 
     constexpr int NP = n_modes(P);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-        const float* bp = u + (size_t)(v * NP + 4 * s) * Kp;
+        const DG_GLOBAL float* bp = gp(u) + (size_t)(v * NP + 4 * s) * Kp;
         dst[s] = (4 * s + g < NP) ? bp[lane_off] : 0.f;
     }
 }
@@ -73,9 +73,13 @@ __device__ __forceinline__ void load_state_var(const float* __restrict__ u, int 
 template <int N, int P>
 struct MfmaA {
     float vol[4], tr[4], sp[4], gx[4], gy[4];
-    __device__ __forceinline__ void init(int g, int j) {
+    // `z` is zero at run time; a caller that loops over several instantiations
+    // passes a loop-variant zero so the table reads stay inside its loop
+    // instead of being hoisted (and held in registers) for every variant.
+    __device__ __forceinline__ void init(int g, int j, int z = 0) {
         constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1, NPT = 3 * NQ1;
-        const TabP<float, P>& T = tab<float, P>();
+        const TabP<float, P>& T =
+            *reinterpret_cast<const TabP<float, P>*>(reinterpret_cast<const float*>(&tab<float, P>()) + z);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             const int m = 4 * s + g;
@@ -105,7 +109,7 @@ struct MfmaTileState {
         for (int q = 0; q < HPW; q++) {
             const int pr = w + 4 * q;
             if (pr < npairs) {
-                const int he = halo[(unsigned)(tile * hcap + (pr / N) * 16 + j)];
+                const int he = gp(halo)[(unsigned)(tile * hcap + (pr / N) * 16 + j)];
                 load_state_var<P>(u, Kp, (unsigned)(g * Kp + he), g, pr % N, hb[q]);
             } else {
 #pragma unroll
@@ -144,17 +148,18 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
     for (int i = threadIdx.x; i < TILE * 3; i += blockDim.x) {
         const int l = i / 3, f = i - 3 * l;
         const unsigned e2 = (unsigned)min(base + l, K - 1);
-        const float* gf = a.geo_f + (size_t)(3 * f) * Kp;
+        const DG_GLOBAL float* gf = gp(a.geo_f) + (size_t)(3 * f) * Kp;
         float4 r;
         r.x = gf[e2];
         r.y = (gf + Kp)[e2];
         r.z = (gf + 2 * (size_t)Kp)[e2];
-        r.w = __int_as_float((int)(mx.fword + (size_t)f * Kp)[e2]);
+        r.w = __int_as_float((int)(gp(mx.fword) + (size_t)f * Kp)[e2]);
         fd[i] = r;
     }
     const unsigned ueu = (unsigned)ee;
-    const float xr = a.geo_e[ueu], yr = (a.geo_e + Kp)[ueu], xs = (a.geo_e + 2 * (size_t)Kp)[ueu];
-    const float ys = (a.geo_e + 3 * (size_t)Kp)[ueu], Jv = (a.geo_e + 4 * (size_t)Kp)[ueu];
+    const DG_GLOBAL float* ge = gp(a.geo_e);
+    const float xr = ge[ueu], yr = (ge + Kp)[ueu], xs = (ge + 2 * (size_t)Kp)[ueu];
+    const float ys = (ge + 3 * (size_t)Kp)[ueu], Jv = (ge + 4 * (size_t)Kp)[ueu];
 
     // ---- phase 1: face traces of the own sub-tile and of this wave's halo pairs ----
 #pragma unroll
@@ -258,21 +263,21 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
         for (int i = 0; i < 4; i++) {
             const bool mv = 4 * g + i < NP;
             const size_t uoff = (size_t)(v * NP + i) * Kp;
-            const float c = (NEED_C && mv) ? (a.c + uoff)[lane_rk] : 0.f;
-            const float aa = (NEED_ACC && mv) ? (a.acc + uoff)[lane_rk] : 0.f;
+            const float c = (NEED_C && mv) ? (gp(a.c) + uoff)[lane_rk] : 0.f;
+            const float aa = (NEED_ACC && mv) ? (gp(a.acc) + uoff)[lane_rk] : 0.f;
             if (mv && valid) {
                 const float k = cj * R[v][i];
                 float o;
                 if constexpr (KIND == ST_RHS) {
                     o = k;
                 } else if constexpr (KIND == ST_RK4_1) {
-                    (a.acc + uoff)[lane_st] = k * (1.f / 6.f);
+                    (gp(a.acc) + uoff)[lane_st] = k * (1.f / 6.f);
                     o = c + 0.5f * k;
                 } else if constexpr (KIND == ST_RK4_2) {
-                    (a.acc + uoff)[lane_st] = aa + k * (1.f / 3.f);
+                    (gp(a.acc) + uoff)[lane_st] = aa + k * (1.f / 3.f);
                     o = c + 0.5f * k;
                 } else if constexpr (KIND == ST_RK4_3) {
-                    (a.acc + uoff)[lane_st] = aa + k * (1.f / 3.f);
+                    (gp(a.acc) + uoff)[lane_st] = aa + k * (1.f / 3.f);
                     o = c + k;
                 } else if constexpr (KIND == ST_RK4_4) {
                     o = c + (aa + k * (1.f / 6.f));
@@ -281,7 +286,7 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
                 } else {
                     o = c + k;
                 }
-                (a.out + uoff)[lane_st] = o;
+                (gp(a.out) + uoff)[lane_st] = o;
                 if (i == 0) c0[v] = o;
             }
         }

@@ -41,7 +41,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
     for (int v = 0; v < N; v++)
 #pragma unroll
-        for (int m = 0; m < NP; m++) u[v][m] = a.u[(size_t)(v * NP + m) * Kp + ee];
+        for (int m = 0; m < NP; m++) u[v][m] = gp(a.u)[(size_t)(v * NP + m) * Kp + ee];
 
     R tot[N][NP];
 #pragma unroll
@@ -51,7 +51,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 
     // ---- volume: sum_q F(U_q) . (metric * grad phi_m) ----
     {
-        const R xr = a.geo_e[ee], yr = a.geo_e[Kp + ee], xs = a.geo_e[2 * Kp + ee], ys = a.geo_e[3 * Kp + ee];
+        const R xr = gp(a.geo_e)[ee], yr = gp(a.geo_e)[Kp + ee], xs = gp(a.geo_e)[2 * Kp + ee], ys = gp(a.geo_e)[3 * Kp + ee];
 #pragma unroll 2
         for (int q = 0; q < NQ; q++) {
             R U[N];
@@ -75,13 +75,13 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
     // ---- the three faces, added in face order as eval_rhs adds s1, s2, s3 ----
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        const int w = a.nbr[f * Kp + ee];
+        const int w = gp(a.nbr)[f * Kp + ee];
         const int nb = w & NB_ELEM_MASK;
         const int fn = (w >> 28) & 3;
         const bool left = (w >> 30) & 1;
-        const R nx = a.geo_f[(3 * f + 0) * Kp + ee];
-        const R ny = a.geo_f[(3 * f + 1) * Kp + ee];
-        const R len = a.geo_f[(3 * f + 2) * Kp + ee];
+        const R nx = gp(a.geo_f)[(3 * f + 0) * Kp + ee];
+        const R ny = gp(a.geo_f)[(3 * f + 1) * Kp + ee];
+        const R len = gp(a.geo_f)[(3 * f + 2) * Kp + ee];
         R FS[NQ1][N];
 #pragma unroll
         for (int j = 0; j < NQ1; j++) {
@@ -103,7 +103,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
                 for (int v = 0; v < N; v++) {
                     Uo[v] += u[v][m] * so;
-                    Un[v] += a.u[(size_t)(v * NP + m) * Kp + nb] * sn;
+                    Un[v] += gp(a.u)[(size_t)(v * NP + m) * Kp + nb] * sn;
                 }
             }
             R UL[N], UR[N];
@@ -133,7 +133,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
     }
 
     // ---- inverse mass (J*I) and the RK stage update ----
-    const R J = a.geo_e[4 * Kp + ee];
+    const R J = gp(a.geo_e)[4 * Kp + ee];
     const R cj = R(1) / J * dt;
     double lam = 0.0;
     if (kind == ST_RHS) {
@@ -141,7 +141,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
             for (int v = 0; v < N; v++)
 #pragma unroll
-                for (int m = 0; m < NP; m++) a.out[(size_t)(v * NP + m) * Kp + e] = cj * tot[v][m];
+                for (int m = 0; m < NP; m++) gp(a.out)[(size_t)(v * NP + m) * Kp + e] = cj * tot[v][m];
     } else {
         R c0[N];
 #pragma unroll
@@ -150,22 +150,22 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
             for (int m = 0; m < NP; m++) {
                 const size_t ix = (size_t)(v * NP + m) * Kp + ee;
                 const R k = cj * tot[v][m];
-                const R c = (kind == ST_RK4_1 || kind == ST_RK2_1) ? u[v][m] : a.c[ix];
+                const R c = (kind == ST_RK4_1 || kind == ST_RK2_1) ? u[v][m] : gp(a.c)[ix];
                 R o;
                 if (kind == ST_RK4_1) {
-                    if (valid) a.acc[ix] = k / R(6);
+                    if (valid) gp(a.acc)[ix] = k / R(6);
                     o = c + R(0.5) * k;
                 } else if (kind == ST_RK4_2 || kind == ST_RK4_3) {
-                    if (valid) a.acc[ix] = a.acc[ix] + k / R(3);
+                    if (valid) gp(a.acc)[ix] = gp(a.acc)[ix] + k / R(3);
                     o = (kind == ST_RK4_2) ? c + R(0.5) * k : c + R(1.0) * k;
                 } else if (kind == ST_RK4_4) {
-                    o = c + (a.acc[ix] + k / R(6));
+                    o = c + (gp(a.acc)[ix] + k / R(6));
                 } else if (kind == ST_RK2_1) {
                     o = c + R(0.5) * k;
                 } else {
                     o = c + k;
                 }
-                if (valid) a.out[ix] = o;
+                if (valid) gp(a.out)[ix] = o;
                 if (m == 0) c0[v] = o;
             }
         if (kind == ST_RK4_4 || kind == ST_RK2_2) {

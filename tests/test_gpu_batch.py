@@ -3,8 +3,9 @@ mixed size, order and physics.
 
 Bars: each solve's result in a mix equals its result alone (bit for bit:
 no cross-solve reductions, per-solve dt); FP32 batch equals the single-solve
-matrix-core path bit for bit; FP64 batch within 1e-12 relative L2 of the
-FP64 oracle.
+path with its default kernel choice (matrix-core tiles for p >= 3 bit for
+bit; lane per element for p <= 2 to 1e-6 relative); FP64 batch within 1e-12 relative
+L2 of the FP64 oracle.
 """
 import numpy as np
 import pytest
@@ -49,8 +50,9 @@ def test_batch_alone_equals_in_mix(prec):
         assert b.audit(k)["t"] == te
 
 
-def test_batch_fp32_equals_single_mfma():
-    cases = mix(seed=9, n=6)
+def test_batch_fp32_equals_single_solve():
+    cases = mix(seed=1, n=8)
+    assert {p for _, _, p, _ in cases} == {1, 2, 3, 4}
     b = dg.Batch(dg.DG_FP32)
     for m, prob, p, te in cases:
         b.add(m, prob, p, te)
@@ -58,13 +60,18 @@ def test_batch_fp32_equals_single_mfma():
     b.run_steps(12, 0.3)
     for k, (m, prob, p, te) in enumerate(cases):
         s = dg.Solver(m, prob, p, dg.DG_FP32)
-        s.set_kernel(2)
+        assert s.kernel() == (2 if p >= 3 else 1)
         s.project_ic()
         s.run_steps(12, 0.3)
         got, want = b.get_state(k), s.get_state()
         err = float(np.abs(got - want).max() / np.abs(want).max())
         print("slot %d p=%d N=%d max rel diff %.3e" % (k, p, prob.physics, err))
-        assert err <= 1e-6
+        if p >= 3:
+            assert np.array_equal(got, want), k
+        else:
+            # same lane body, but with the stage kind fixed at compile time the
+            # compiler contracts a few multiply-adds differently: ulp-level
+            assert err <= 1e-6, k
 
 
 def test_batch_fp64_vs_oracle():
@@ -84,7 +91,7 @@ def test_batch_fp64_vs_oracle():
 
 
 def test_batch_fp32_mfma_low_order_vs_oracle():
-    """The matrix-core body at p = 1, 2 (used by FP32 batches)."""
+    """The matrix-core body at p = 1, 2 (selectable for a single solve)."""
     for p in (1, 2):
         m = dg.Mesh.periodic_jittered(12, 12, 0.0, 10.0, 0.0, 10.0, 0.1, 4)
         prob = dg.euler_vortex(((4.0, 5.0, 5.0),))
