@@ -118,11 +118,12 @@ def rhs(V, elem_s, sides, side_int, period, u, p, phys, coeff, vol_rule, face_ru
             vb = verts[nb]
             mid_nb = 0.5 * (vb[nbf] + vb[(nbf + 1) % 3])
             mid = 0.5 * (a + b)
+            # the neighbour's copy of this face lies at mid - shift
             shift = np.zeros(2)
             if Lx > 0:
-                shift[0] = -Lx * np.round((mid[0] - mid_nb[0]) / Lx)
+                shift[0] = Lx * np.round((mid[0] - mid_nb[0]) / Lx)
             if Ly > 0:
-                shift[1] = -Ly * np.round((mid[1] - mid_nb[1]) / Ly)
+                shift[1] = Ly * np.round((mid[1] - mid_nb[1]) / Ly)
             for xi, wi in zip(xf, wf):
                 x = mid + 0.5 * xi * d
                 uin = trace(e, x, np.zeros(2))
