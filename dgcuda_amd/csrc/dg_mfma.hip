@@ -39,8 +39,10 @@ hipError_t mfma_layout_probe(float* host_out) {
 }
 
 // occupancy target (waves per SIMD) the register allocation is held to
+// at least 3 workgroups (12 wavefronts) per CU: the register budget the
+// p = 4 body fits without spills, and the faster setting on MI355X
 #ifndef DGMI_MFMA_MINW
-#define DGMI_MFMA_MINW 2
+#define DGMI_MFMA_MINW 3
 #endif
 
 template <int N, int P, int KIND>
@@ -66,17 +68,21 @@ __global__ void __launch_bounds__(256, DGMI_MFMA_MINW) k_stage_mfma(StageArgs<fl
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = lane >> 4, j = lane & 15;
-    MfmaA<N, P> A;
-    A.init(g, j);
-    MfmaTileState<N, P> cur, nxt;
-    if (t < tend) cur.load(a.u, a.K, a.Kp, mx.halo, mx.hcap, t, w, g, j);
     double lam = 0.0;
+    __shared__ float4 tabA[MFMA_A_SLOTS * 64];
+    MfmaTileState<N, P> st;
+    int hid[MfmaTileState<N, P>::HPW];
+    if (t < tend) {
+        MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, t, w, j, hid);
+        st.load_data(a.u, a.K, a.Kp, mx.hcap, t, w, g, j, hid);
+    }
+    if (w == 0) mfma_a_to_lds<N, P>(tabA, lane);
+    __syncthreads();
     for (; t < tend; t += stride) {
         const int tn = t + stride;
-        if (tn < tend) nxt.load(a.u, a.K, a.Kp, mx.halo, mx.hcap, tn, w, g, j);
-        lam = fmax(lam, mfma_process_tile<N, P, KIND>(a, mx, A, cur, t, fd, ts));
-        cur = nxt;
+        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND>(a, mx, tabA, st, hid, t, tn < tend ? tn : -1, fd, ts));
     }
+
     if constexpr (LAST) {
         const double m2 = block_max(lam, shmax);
         if (threadIdx.x == 0) lam_commit(m2, a.lam_bits);

@@ -21,6 +21,7 @@
 // identical operands, so the two contributions are bit-identical with
 // opposite signs: race-free, no atomics, no face buffers in HBM.
 #pragma once
+
 #include "dg_kernels.h"
 
 namespace dgmi {
@@ -41,8 +42,7 @@ __host__ __device__ constexpr size_t mfma_tile_lds_bytes(int N, int hcap) {
 }
 
 // Global addresses are split into a wavefront-uniform base (scalar
-// registers) and one per-lane 32-bit unsigned offset, so loads take the
-// scalar-base form and per-load address arithmetic stays off the VALU.
+// registers) and one per-lane 32-bit unsigned offset.
 template <int N, int P>
 __device__ __forceinline__ void load_state(const float* __restrict__ u, int Kp, unsigned lane_off, int g,
                                            float (&dst)[N][4]) {
@@ -56,10 +56,9 @@ __device__ __forceinline__ void load_state(const float* __restrict__ u, int Kp, 
         }
 }
 
-// one variable of one halo sub-tile: the B operand of its trace product
 template <int P>
 __device__ __forceinline__ void load_state_var(const float* __restrict__ u, int Kp, unsigned lane_off, int g, int v,
-                                               float (&dst)[4]) {
+                                               int nrows, float (&dst)[4]) {
     constexpr int NP = n_modes(P);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
@@ -93,6 +92,22 @@ struct MfmaA {
     }
 };
 
+// Slots of the per-lane A operands in a workgroup's LDS table [5][64] of
+// float4 (the pipelined kernel reads them where used instead of holding 20
+// registers).
+enum { MFMA_A_VOL = 0, MFMA_A_TR = 1, MFMA_A_SP = 2, MFMA_A_GX = 3, MFMA_A_GY = 4, MFMA_A_SLOTS = 5 };
+
+template <int N, int P>
+__device__ __forceinline__ void mfma_a_to_lds(float4* tabA, int lane) {
+    MfmaA<N, P> A;
+    A.init(lane >> 4, lane & 15);
+    tabA[MFMA_A_VOL * 64 + lane] = make_float4(A.vol[0], A.vol[1], A.vol[2], A.vol[3]);
+    tabA[MFMA_A_TR * 64 + lane] = make_float4(A.tr[0], A.tr[1], A.tr[2], A.tr[3]);
+    tabA[MFMA_A_SP * 64 + lane] = make_float4(A.sp[0], A.sp[1], A.sp[2], A.sp[3]);
+    tabA[MFMA_A_GX * 64 + lane] = make_float4(A.gx[0], A.gx[1], A.gx[2], A.gx[3]);
+    tabA[MFMA_A_GY * 64 + lane] = make_float4(A.gy[0], A.gy[1], A.gy[2], A.gy[3]);
+}
+
 // State operands of one tile held by one wavefront: its own sub-tile, and
 // its share of the halo, dealt as (halo sub-tile, variable) pairs
 // round-robin to the 4 wavefronts.
@@ -101,23 +116,98 @@ struct MfmaTileState {
     static constexpr int HPW = (4 * N + 3) / 4;  // most halo pairs per wave (hcap <= 64)
     float ub[N][4];
     float hb[HPW][4];
-    __device__ __forceinline__ void load(const float* __restrict__ u, int K, int Kp, const int* __restrict__ halo,
-                                         int hcap, int tile, int w, int g, int j) {
-        load_state<N, P>(u, Kp, (unsigned)(g * Kp + min(tile * MFMA_TILE + w * 16 + j, K - 1)), g, ub);
+    // Halo element ids of this wave's pairs.  A halo state load depends on
+    // its id, and waiting for an id waits for every load issued before it,
+    // so the ids are requested ahead of the state loads that use them.
+    __device__ __forceinline__ static void load_ids(const int* __restrict__ halo, int hcap, int tile, int w, int j,
+                                                   int (&hid)[HPW]) {
         const int npairs = (hcap >> 4) * N;
 #pragma unroll
         for (int q = 0; q < HPW; q++) {
             const int pr = w + 4 * q;
+            hid[q] = (pr < npairs) ? gp(halo)[(unsigned)(tile * hcap + (pr / N) * 16 + j)] : 0;
+        }
+    }
+    __device__ __forceinline__ void load_data(const float* __restrict__ u, int K, int Kp, int hcap, int tile, int w,
+                                              int g, int j, const int (&hid)[HPW]) {
+        load_state<N, P>(u, Kp, (unsigned)(g * Kp + min(tile * MFMA_TILE + w * 16 + j, K - 1)), g, ub);
+        const int npairs = (hcap >> 4) * N;
+        constexpr int NP = n_modes(P);
+#pragma unroll
+        for (int q = 0; q < HPW; q++) {
+            const int pr = w + 4 * q;
             if (pr < npairs) {
-                const int he = gp(halo)[(unsigned)(tile * hcap + (pr / N) * 16 + j)];
-                load_state_var<P>(u, Kp, (unsigned)(g * Kp + he), g, pr % N, hb[q]);
+                load_state_var<P>(u, Kp, (unsigned)(g * Kp + hid[q]), g, pr % N, N * NP, hb[q]);
             } else {
 #pragma unroll
                 for (int s = 0; s < 4; s++) hb[q][s] = 0.f;
             }
         }
     }
+    __device__ __forceinline__ void load(const float* __restrict__ u, int K, int Kp, const int* __restrict__ halo,
+                                         int hcap, int tile, int w, int g, int j) {
+        int hid[HPW];
+        load_ids(halo, hcap, tile, w, j, hid);
+        load_data(u, K, Kp, hcap, tile, w, g, j, hid);
+    }
 };
+
+// The RK update runs in the product output layout (lane j + 16g holds
+// modes 4g..4g+3 of element j) while states are read in the operand layout
+// (lane j + 16k holds mode 4s+k of block s).  For a stage whose input is
+// the RK base state itself (RK4 stage 1, RK2 stage 1) the base state is put
+// in the output layout from the registers already loaded, with products
+// against 0/1 selector operands (exact: one product of 1*x per entry), so
+// it is not read from memory a second time.
+template <int N>
+__device__ __forceinline__ void operand_to_rows(const float (&ub)[N][4], int lane, float (&rows)[N][4]) {
+    const int i = lane & 15, k = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < N; v++) {
+        floatx4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; s++) d = mfma4(i == 4 * s + k ? 1.f : 0.f, ub[v][s], d);
+#pragma unroll
+        for (int r = 0; r < 4; r++) rows[v][r] = d[r];
+    }
+}
+
+// RK4 on the matrix-core path keeps acc = u_n + k1/6 + k2/3 + k3/3 (the
+// base state folded in), so the last stage writes acc + k4/6 without
+// reading u_n; classical RK4 with the FP32 sums grouped differently from
+// the reference's c + (k1/6 + k2/3 + k3/3 + k4/6).
+template <int KIND>
+struct RkNeeds {
+    static constexpr bool C_FROM_INPUT = (KIND == ST_RK4_1 || KIND == ST_RK2_1);
+    static constexpr bool LOAD_C = (KIND == ST_RK4_2 || KIND == ST_RK4_3 || KIND == ST_RK2_2);
+    static constexpr bool LOAD_ACC = (KIND == ST_RK4_2 || KIND == ST_RK4_3 || KIND == ST_RK4_4);
+};
+
+template <int KIND>
+__device__ __forceinline__ float rk_update(float c, float aa, float k, float* acc_out) {
+    if constexpr (KIND == ST_RHS) {
+        return k;
+    } else if constexpr (KIND == ST_RK4_1) {
+        *acc_out = c + k * (1.f / 6.f);
+        return c + 0.5f * k;
+    } else if constexpr (KIND == ST_RK4_2) {
+        *acc_out = aa + k * (1.f / 3.f);
+        return c + 0.5f * k;
+    } else if constexpr (KIND == ST_RK4_3) {
+        *acc_out = aa + k * (1.f / 3.f);
+        return c + k;
+    } else if constexpr (KIND == ST_RK4_4) {
+        return aa + k * (1.f / 6.f);
+    } else if constexpr (KIND == ST_RK2_1) {
+        return c + 0.5f * k;
+    } else {
+        return c + k;
+    }
+}
+template <int KIND>
+constexpr bool rk_writes_acc() {
+    return KIND == ST_RK4_1 || KIND == ST_RK4_2 || KIND == ST_RK4_3;
+}
 
 // Processes one tile (all 4 wavefronts of the workgroup call it).  Returns
 // the lane's maximum cell wave speed of the new state for the last stage
@@ -130,8 +220,6 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
     constexpr int TILE = MFMA_TILE;
     constexpr int TS = 17 * N;
     constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
-    constexpr bool NEED_ACC = (KIND == ST_RK4_2 || KIND == ST_RK4_3 || KIND == ST_RK4_4);
-    constexpr bool NEED_C = (KIND != ST_RHS);
     constexpr int HPW = MfmaTileState<N, P>::HPW;
     static_assert(NP <= 16 && NQ <= 16 && NPT <= 16, "order too high for 16x16 tiles");
     const int K = a.K, Kp = a.Kp;
@@ -252,7 +340,227 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
     }
 
     // inverse mass and the RK stage update; lane holds modes m = 4g+i
+    using RK = RkNeeds<KIND>;
     const unsigned lane_rk = (unsigned)(4 * g * Kp + ee);
+    const unsigned lane_st = (unsigned)(4 * g * Kp + e);
+    const float cj = __builtin_amdgcn_rcpf(Jv) * dt;
+    float cin[N][4];
+    if constexpr (RK::C_FROM_INPUT) operand_to_rows<N>(st.ub, lane, cin);
+    float c0[N];
+    double lam = 0.0;
+#pragma unroll
+    for (int v = 0; v < N; v++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const bool mv = 4 * g + i < NP;
+            const size_t uoff = (size_t)(v * NP + i) * Kp;
+            float c = 0.f;
+            if constexpr (RK::C_FROM_INPUT) c = cin[v][i];
+            if constexpr (RK::LOAD_C) c = mv ? (gp(a.c) + uoff)[lane_rk] : 0.f;
+            const float aa = (RK::LOAD_ACC && mv) ? (gp(a.acc) + uoff)[lane_rk] : 0.f;
+            if (mv && valid) {
+                float an = 0.f;
+                const float o = rk_update<KIND>(c, aa, cj * R[v][i], &an);
+                if constexpr (rk_writes_acc<KIND>()) (gp(a.acc) + uoff)[lane_st] = an;
+                (gp(a.out) + uoff)[lane_st] = o;
+                if (i == 0) c0[v] = o;
+            }
+        }
+    if constexpr (LAST) {
+        if (g == 0 && valid) {
+            double Uc[4];
+            const double b0 = tab<double, P>().B[0][0];
+#pragma unroll
+            for (int v = 0; v < N; v++) Uc[v] = (double)c0[v] * b0;
+            lam = cell_speed<N>(a.phys, Uc);
+        }
+    }
+    __syncthreads();  // the next tile reuses the LDS
+    return lam;
+}
+
+// Geometry of one tile: the (face normal, length, packed neighbour word)
+// entry this thread stores to LDS (threads 0..191 hold one of the 64 x 3
+// entries) and the affine map of the lane's own element.
+struct TileGeo {
+    float4 f;
+    float xr, yr, xs, ys, J;
+    __device__ __forceinline__ void load(const StageArgs<float>& a, const MfmaArgs& mx, int tile, int el) {
+        const int K = a.K, Kp = a.Kp;
+        const int i = threadIdx.x;
+        if (i < MFMA_TILE * 3) {
+            const int l = i / 3, fc = i - 3 * l;
+            const unsigned e2 = (unsigned)min(tile * MFMA_TILE + l, K - 1);
+            const DG_GLOBAL float* gf = gp(a.geo_f) + (size_t)(3 * fc) * Kp;
+            f.x = gf[e2];
+            f.y = (gf + Kp)[e2];
+            f.z = (gf + 2 * (size_t)Kp)[e2];
+            f.w = __int_as_float((int)(gp(mx.fword) + (size_t)fc * Kp)[e2]);
+        }
+        const unsigned ueu = (unsigned)min(tile * MFMA_TILE + el, K - 1);
+        const DG_GLOBAL float* ge = gp(a.geo_e);
+        xr = ge[ueu];
+        yr = (ge + Kp)[ueu];
+        xs = (ge + 2 * (size_t)Kp)[ueu];
+        ys = (ge + 3 * (size_t)Kp)[ueu];
+        J = (ge + 4 * (size_t)Kp)[ueu];
+    }
+};
+
+// Software-pipelined body of the persistent single-solve kernel: `st`
+// holds this tile's state on entry and `hid` the halo ids of tile `next`
+// are requested at the start.  Once the volume interpolation has consumed
+// the state, the tile's RK registers (c, acc) are requested first and then
+// the state of tile `next` (if >= 0) into the same registers, so the RK
+// update waits only for its own loads and the next tile's loads overlap the
+// flux work, the update and the stores.  The vector-memory counter retires
+// in issue order, hence the ordering (and the scheduling barriers that
+// keep it).  The A operands are read from the LDS table `tabA`.
+template <int N, int P, int KIND>
+__device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>& a, const MfmaArgs& mx,
+                                                         const float4* tabA, MfmaTileState<N, P>& st,
+                                                         int (&hid)[MfmaTileState<N, P>::HPW], int tile, int next,
+                                                         float4* fd, float* ts) {
+    constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1, NPT = 3 * NQ1;
+    constexpr int TILE = MFMA_TILE;
+    constexpr int TS = 17 * N;
+    constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
+    using RK = RkNeeds<KIND>;
+    constexpr int HPW = MfmaTileState<N, P>::HPW;
+    static_assert(NP <= 16 && NQ <= 16 && NPT <= 16, "order too high for 16x16 tiles");
+    const int K = a.K, Kp = a.Kp;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    const int npairs = (mx.hcap >> 4) * N;
+    const float dt = (float)a.ctrl->dt;
+    const int base = tile * TILE;
+    const int el = w * 16 + j;
+    const int e = base + el;
+    const bool valid = e < K;
+    const int ee = valid ? e : K - 1;
+
+    // the tile's geometry, requested first: it is needed only after phase 1
+    TileGeo gc;
+    gc.load(a, mx, tile, el);
+    if (next >= 0) MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, next, w, j, hid);
+    const float xr = gc.xr, yr = gc.yr, xs = gc.xs, ys = gc.ys, Jv = gc.J;
+
+    // ---- phase 1: face traces of the own sub-tile and of this wave's halo pairs ----
+    const float4 Atr = tabA[MFMA_A_TR * 64 + lane];
+#pragma unroll
+    for (int v = 0; v < N; v++) {
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; s++) acc = mfma4(Atr[s], st.ub[v][s], acc);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (4 * g + i < NPT) ts[el * TS + (4 * g + i) * N + v] = acc[i];
+    }
+#pragma unroll
+    for (int q = 0; q < HPW; q++) {
+        const int pr = w + 4 * q;
+        if (pr < npairs) {
+            const int slot = TILE + (pr / N) * 16 + j, v = pr % N;
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc = mfma4(Atr[s], st.hb[q][s], acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (4 * g + i < NPT) ts[slot * TS + (4 * g + i) * N + v] = acc[i];
+        }
+    }
+    if (threadIdx.x < TILE * 3) fd[threadIdx.x] = gc.f;
+    __syncthreads();
+
+    // ---- phase 2: volume, faces, inverse mass, RK update ----
+    floatx4 U[N];
+    const float4 Avol = tabA[MFMA_A_VOL * 64 + lane];
+#pragma unroll
+    for (int v = 0; v < N; v++) {
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; s++) acc = mfma4(Avol[s], st.ub[v][s], acc);
+        U[v] = acc;
+    }
+    // RK registers of this tile, then the next tile's state (st is dead
+    // after this block).  The loads are unconditional (row clamped into the
+    // array for the padding modes m >= n_p, whose values are never used), so
+    // no branches sit between them and the waits that follow.
+    float cr[N][4], ar[N][4];
+    if constexpr (RK::C_FROM_INPUT) operand_to_rows<N>(st.ub, lane, cr);
+#pragma unroll
+    for (int v = 0; v < N; v++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const unsigned row = (unsigned)min(v * NP + 4 * g + i, N * NP - 1);
+            const unsigned ix = row * (unsigned)Kp + (unsigned)ee;
+            if constexpr (!RK::C_FROM_INPUT) cr[v][i] = RK::LOAD_C ? gp(a.c)[ix] : 0.f;
+            ar[v][i] = RK::LOAD_ACC ? gp(a.acc)[ix] : 0.f;
+        }
+    __builtin_amdgcn_sched_barrier(0);
+    if (next >= 0) {
+        st.load_data(a.u, K, Kp, mx.hcap, next, w, g, j, hid);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    floatx4 R[N];
+#pragma unroll
+    for (int v = 0; v < N; v++) R[v] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const float4 Agx = tabA[MFMA_A_GX * 64 + lane], Agy = tabA[MFMA_A_GY * 64 + lane];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        // volume point q = 4g+i: flux, metric, reduction step i
+        float Uq[N], fx[N], fy[N];
+#pragma unroll
+        for (int v = 0; v < N; v++) Uq[v] = U[v][i];
+        const bool qv = (4 * g + i) < NQ;
+        if (N == 4 && !qv) Uq[0] = 1.f;  // padding point: keep the flux finite
+        flux<N, float>(a.phys, Uq, fx, fy);
+#pragma unroll
+        for (int v = 0; v < N; v++) {
+            const float fr = qv ? fx[v] * ys - fy[v] * xs : 0.f;
+            const float fs = qv ? fy[v] * xr - fx[v] * yr : 0.f;
+            R[v] = mfma4(Agx[i], fr, R[v]);
+            R[v] = mfma4(Agy[i], fs, R[v]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        // own face point pt = 4g+i: LLF flux in the side's orientation
+        const int pt = 4 * g + i;
+        float G[N];
+        if (pt < NPT) {
+            const int f = pt / NQ1, jo = pt - f * NQ1;
+            const float4 r = fd[el * 3 + f];
+            const int wd = __float_as_int(r.w);
+            const int slot = wd & 0xFFF, fn = (wd >> 12) & 3;
+            const bool left = (wd >> 14) & 1;
+            const float* own = ts + el * TS + pt * N;
+            const float* nbt = ts + slot * TS + (fn * NQ1 + NQ1 - 1 - jo) * N;
+            float UL[N], UR[N];
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                const float uo = own[v], un = nbt[v];
+                UL[v] = left ? uo : un;
+                UR[v] = left ? un : uo;
+            }
+            float fxl[N], fyl[N], fxr[N], fyr[N];
+            flux<N, float>(a.phys, UL, fxl, fyl);
+            flux<N, float>(a.phys, UR, fxr, fyr);
+            const float lm = llf_speed<N, float>(a.phys, UL, UR, r.x, r.y);
+            const float coef = left ? -0.5f * r.z : 0.5f * r.z;
+#pragma unroll
+            for (int v = 0; v < N; v++)
+                G[v] = coef * (0.5f * ((fxl[v] + fxr[v]) * r.x + (fyl[v] + fyr[v]) * r.y + lm * (UL[v] - UR[v])));
+        } else {
+#pragma unroll
+            for (int v = 0; v < N; v++) G[v] = 0.f;
+        }
+        const float4 Asp = tabA[MFMA_A_SP * 64 + lane];
+#pragma unroll
+        for (int v = 0; v < N; v++) R[v] = mfma4(Asp[i], G[v], R[v]);
+    }
+
+    // inverse mass and the RK stage update; lane holds modes m = 4g+i
     const unsigned lane_st = (unsigned)(4 * g * Kp + e);
     const float cj = __builtin_amdgcn_rcpf(Jv) * dt;
     float c0[N];
@@ -263,29 +571,12 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
         for (int i = 0; i < 4; i++) {
             const bool mv = 4 * g + i < NP;
             const size_t uoff = (size_t)(v * NP + i) * Kp;
-            const float c = (NEED_C && mv) ? (gp(a.c) + uoff)[lane_rk] : 0.f;
-            const float aa = (NEED_ACC && mv) ? (gp(a.acc) + uoff)[lane_rk] : 0.f;
+            const float c = cr[v][i];
+            const float aa = ar[v][i];
             if (mv && valid) {
-                const float k = cj * R[v][i];
-                float o;
-                if constexpr (KIND == ST_RHS) {
-                    o = k;
-                } else if constexpr (KIND == ST_RK4_1) {
-                    (gp(a.acc) + uoff)[lane_st] = k * (1.f / 6.f);
-                    o = c + 0.5f * k;
-                } else if constexpr (KIND == ST_RK4_2) {
-                    (gp(a.acc) + uoff)[lane_st] = aa + k * (1.f / 3.f);
-                    o = c + 0.5f * k;
-                } else if constexpr (KIND == ST_RK4_3) {
-                    (gp(a.acc) + uoff)[lane_st] = aa + k * (1.f / 3.f);
-                    o = c + k;
-                } else if constexpr (KIND == ST_RK4_4) {
-                    o = c + (aa + k * (1.f / 6.f));
-                } else if constexpr (KIND == ST_RK2_1) {
-                    o = c + 0.5f * k;
-                } else {
-                    o = c + k;
-                }
+                float an = 0.f;
+                const float o = rk_update<KIND>(c, aa, cj * R[v][i], &an);
+                if constexpr (rk_writes_acc<KIND>()) (gp(a.acc) + uoff)[lane_st] = an;
                 (gp(a.out) + uoff)[lane_st] = o;
                 if (i == 0) c0[v] = o;
             }
