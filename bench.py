@@ -30,12 +30,18 @@ def vortices(seed, n=3, L=10.0):
     return tuple((float(rng.uniform(2, 5)), float(rng.uniform(0, L)), float(rng.uniform(0, L))) for _ in range(n))
 
 
-def algorithmic_bytes_per_elem(N, n_p, s):
-    """Bytes one RK4 stage must move per element, averaged over the 4 stages:
-    state and RK registers 4.25*N*n_p*s (stage 1 reads c, writes acc and the
-    next input; stages 2-3 read input, c, acc and write acc and the next input;
-    stage 4 reads input, c, acc and writes c) plus the element's geometry
-    (xr, yr, xs, ys, J; per face nx, ny, len; 3 packed neighbour ints)."""
+def algorithmic_bytes_per_elem(N, n_p, s, kernel):
+    """Bytes one RK4 stage must move per element, averaged over the 4 stages
+    (each input read once, each output written once; halo re-reads are not
+    counted).  Matrix-core path (kernel 2/3): state passes 4.0*N*n_p*s
+    (stage 1 reads u_n and writes acc and the next input; stages 2-3 read
+    the input, u_n and acc and write acc and the next input; stage 4 reads
+    the input and acc, which carries u_n, and writes u_n+1), plus geometry
+    14*s (xr, yr, xs, ys, J; per face nx, ny, len) and 3 16-bit neighbour
+    words.  Lane-per-element path (kernel 1): the reference's grouping, where
+    stage 4 also reads u_n (4.25 passes), and 3 32-bit neighbour words."""
+    if kernel >= 2:
+        return 4.0 * N * n_p * s + 14 * s + 3 * 2
     return 4.25 * N * n_p * s + 14 * s + 3 * 4
 
 
@@ -61,12 +67,18 @@ def cpu_baseline(p, seed, n_patch=64, steps=40):
                        "%.1f s" % (n_patch, n_patch, m.num_elem, p, steps, el))
 
 
-def load_traffic(p):
-    path = os.path.join(ROOT, "profiles", "traffic_euler_p%d_524k.json" % p)
-    if os.path.exists(path):
-        with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
-    return None
+def load_traffic(args, kernel):
+    """Measured HBM bytes per stage launch (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+    calibrated on a copy kernel), committed under profiles/ for the default
+    workload; None for any other configuration."""
+    if (args.p, args.n, args.precision, args.seed) != (4, 512, "fp32", 2024):
+        return None
+    path = os.path.join(ROOT, "profiles", "traffic_euler_p4_524k.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch") if d.get("kernel") == kernel else None
 
 
 def main():
@@ -124,11 +136,11 @@ def main():
     # dominant kernel: the fused stage kernel, timed per launch with HIP
     # events on the solver's stream
     tk = s.time_stage_kernels(20, 0.3)
-    bpe = algorithmic_bytes_per_elem(N, n_p, s_bytes)
+    bpe = algorithmic_bytes_per_elem(N, n_p, s_bytes, s.kernel())
     bytes_launch = bpe * K
     achieved = bytes_launch / (tk["stage_ms"] * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                frac=round(achieved / PEAK_HBM_GBS, 4), traffic=load_traffic(args.p),
+                frac=round(achieved / PEAK_HBM_GBS, 4), traffic=load_traffic(args, s.kernel()),
                 kernel="stage (fused volume + LLF faces + M^-1 + RK update)",
                 stage_ms=round(tk["stage_ms"], 5), dt_kernel_ms=round(tk["dt_ms"], 5),
                 algorithmic_bytes_per_launch=int(bytes_launch), bytes_per_dof_stage=round(bpe / (n_p * N), 3))
