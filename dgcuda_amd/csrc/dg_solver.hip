@@ -7,6 +7,10 @@
 // fused launch per stage; dt never leaves the device, and the host only
 // synchronises once per chunk of steps to test for t >= t_end.
 #include <algorithm>
+#include <cstdlib>
+#include <cstdio>
+#include <tuple>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -89,6 +93,16 @@ Status check_problem(const dg_problem_desc& d, int p, int precision) {
     if (d.physics == DG_PHYS_EULER && !(d.gamma > 1.0)) return Status::err(DG_ERR_ARG, "gamma must exceed 1");
     if (d.quirks & ~3) return Status::err(DG_ERR_ARG, "unknown quirk bits");
     return Status::ok();
+}
+
+// process-wide cache of automatic kernel choices
+static std::mutex& tune_mu() {
+    static std::mutex m;
+    return m;
+}
+static std::map<std::tuple<int, int, int, int>, int>& tune_cache() {
+    static std::map<std::tuple<int, int, int, int>, int> c;
+    return c;
 }
 
 template <typename R>
@@ -263,15 +277,14 @@ class SolverT : public Solver {
         return Status::ok();
     }
 
-    Status choose_shape(int req) {
-        kernel_req = req;
-        int kind = req;
+    // Launch shape of kernel kind 1 (lane per element), 2 (persistent
+    // matrix-core tiles) or 3 (one matrix-core tile per workgroup).
+    Status make_shape(int kind, LaunchShape& s) {
         const bool mf = sizeof(R) == 4 && mfma_supported(N, p);
-        if (kind == 0) kind = (mf && p >= 3) ? 2 : 1;
         if ((kind == 2 || kind == 3) && !mf)
             return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs FP32 and p = 1..4");
         if (kind < 1 || kind > 3) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1, 2 or 3");
-        LaunchShape s;
+        s = LaunchShape{};
         s.kind = kind;
         s.tile_elems = (kind == 1) ? VALU_TILE_ELEMS : MFMA_TILE_ELEMS;
         s.ntiles = (K + s.tile_elems - 1) / s.tile_elems;
@@ -279,19 +292,89 @@ class SolverT : public Solver {
         s.grid = (s.ntiles + 7) / 8 * 8;
         if (kind >= 2) {
             Status st = build_mfma_tiles(s.tile_elems);
-            if (st && kind == 2) {
+            if (!st) return st;
+            if (kind == 2) {
                 // persistent: as many workgroups as fit on the device at once
                 const int res = mfma_resident_groups(N, p, mx.hcap);
                 if (res >= 8) s.grid = std::min(s.grid, res / 8 * 8);
             }
-            if (!st) {
-                if (req != 0) return st;
-                // automatic choice falls back to the VALU kernel
-                s.kind = 1;
-                s.tile_elems = VALU_TILE_ELEMS;
-                s.ntiles = (K + s.tile_elems - 1) / s.tile_elems;
-                s.grid = (s.ntiles + 7) / 8 * 8;
+        }
+        return Status::ok();
+    }
+
+    // Automatic kernel choice: time every kind this solve supports on its
+    // own buffers (3 right-hand-side launches after a warm-up, HIP events)
+    // and take the fastest; cached per (physics, p, precision, log2 K).
+    // FP64 has one kernel.  The candidates agree to the stated tolerances,
+    // so the choice changes speed, not results beyond FP32 rounding.
+    Status autotune(int* out_kind) {
+        *out_kind = 1;
+        if (!(sizeof(R) == 4 && mfma_supported(N, p))) return Status::ok();
+        int lk = 0;
+        while (lk < 30 && (1 << (lk + 1)) <= K) lk++;
+        const auto key = std::make_tuple(N, p, (int)sizeof(R), lk);
+        {
+            std::lock_guard<std::mutex> lk_(tune_mu());
+            auto it = tune_cache().find(key);
+            if (it != tune_cache().end()) {
+                *out_kind = it->second;
+                return Status::ok();
             }
+        }
+        HIPST(hipMemsetAsync(c, 0, state_elems() * sizeof(R), stream));
+        HIPST(launch_set_dt(ctrl, 0.0, 0, stream));  // activates the stage kernels
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPST(hipEventCreate(&e0));
+        HIPST(hipEventCreate(&e1));
+        float best = 0.f;
+        int best_kind = 1;
+        const bool log = getenv("DGMI_TUNE_LOG") != nullptr;
+        Status st;
+        for (int kind : {1, 2, 3}) {
+            LaunchShape cand;
+            if (!make_shape(kind, cand)) continue;
+            shape = cand;
+            if (!(st = stage(ST_RHS, c, c, ub))) break;
+            if (hipEventRecord(e0, stream) != hipSuccess) break;
+            for (int r = 0; r < 3 && st; r++) st = stage(ST_RHS, c, c, ub);
+            if (!st) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                st = Status::err(DG_ERR_HIP, "autotune timing failed");
+                break;
+            }
+            if (log) fprintf(stderr, "dgmi autotune N=%d p=%d K=%d kind %d: %.4f ms per launch\n", N, p, K, kind, ms / 3);
+            if (kind == 1 || ms < best) {
+                best = ms;
+                best_kind = kind;
+            }
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        if (!st) return st;
+        lam_valid = false;  // set_dt cleared the accumulated wave speed
+        {
+            std::lock_guard<std::mutex> lk_(tune_mu());
+            tune_cache()[key] = best_kind;
+        }
+        *out_kind = best_kind;
+        return Status::ok();
+    }
+
+    Status choose_shape(int req) {
+        kernel_req = req;
+        int kind = req;
+        if (kind == 0) {
+            Status st = autotune(&kind);
+            if (!st) return st;
+        }
+        LaunchShape s;
+        Status st = make_shape(kind, s);
+        if (!st) {
+            if (req != 0) return st;
+            // automatic choice falls back to the lane kernel
+            if (!(st = make_shape(1, s))) return st;
         }
         shape = s;
         lam_valid = false;

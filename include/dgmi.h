@@ -25,7 +25,9 @@
  *                        src/2d/conservation/time_integrator.cu:221,466 -> dg_integrate
  *   eval_u               src/2d/conservation/conserv_kernels.cu:818 -> dg_eval_vertices
  *   free_gpu             src/2d/conservation/conserv.cu:323   -> dg_solver_destroy
- *   run_dgcuda           src/2d/conservation/main.cu:8        -> the dgmi command-line driver
+ *   run_dgcuda           src/2d/conservation/main.cu:8        -> the caller's own driver: read or generate
+ *                        the mesh, dg_solver_create, dg_project_ic, dg_integrate (no
+ *                        command-line driver is part of the library)
  *
  * State layout at the boundary ("reference layout"): c[var][mode][elem] in
  * the caller's element numbering, index num_elem*n_p*var + mode*num_elem + elem

@@ -53,8 +53,28 @@ def test_mfma_run_vs_oracle(kind):
     assert np.all(np.abs(np.array(a1) - a0) <= 2e-6 * np.abs(a0) + 1e-9)
 
 
-def test_mfma_auto_selected_for_fp32_p4():
-    s = dg.Solver(mesh(8), problem_for(4), 4, dg.DG_FP32)
+def test_autotuner_picks_matrix_cores_for_fp32_p4():
+    """Automatic choice times the candidate kernels; at 32,768 triangles and
+    p = 4 the matrix-core kernel wins by a wide margin (the lane kernel
+    spills at this order).  FP64 has the lane kernel only."""
+    m = mesh(128)
+    s = dg.Solver(m, problem_for(4), 4, dg.DG_FP32)
     assert s.kernel() in (2, 3)
     s64 = dg.Solver(mesh(8), problem_for(4), 4, dg.DG_FP64)
     assert s64.kernel() == 1
+
+
+def test_autotuned_solver_runs_correctly():
+    """Whatever kernel the tuner picked, a run from a projected state matches
+    the oracle (the tuning launches leave no trace in the solve)."""
+    m = mesh(10)
+    prob = problem_for(4)
+    for p in (1, 2, 3, 4):
+        s = dg.Solver(m, prob, p, dg.DG_FP32)
+        s.project_ic()
+        s.integrate(0.2, 0.3)
+        o = orc.Oracle.from_mesh(m, p, prob)
+        ref, _, _ = o.integrate(o.project_ic(orc.ic_array(prob, m.period)), 0.2, 0.3)
+        err = orc.rel_l2(s.get_state(), ref)
+        print("p=%d kernel %d rel L2 %.3e" % (p, s.kernel(), err))
+        assert err <= 1e-5
