@@ -5,7 +5,10 @@ oracle's throughput beside it.
 
   python bench.py [--gpus 1] [--steps 200] [--warmup 10]
 
-Prints one JSON line.  A step is one classical RK4 step (4 stages) of the
+Prints one JSON line.  Beside the headline workload it carries the ragged
+batch ("batch": 64 solves in one launch per RK stage vs one at a time) and
+the serving loop ("serving": served vs plain solves/s and p50/p99 latency
+on seeded Poisson request streams).  A step is one classical RK4 step (4 stages) of the
 whole mesh with the CFL time step computed on the device; the timed region
 replays the captured step graph `--steps` times with the state resident in
 HBM.
@@ -81,6 +84,45 @@ def load_traffic(args, kernel):
     return d.get("bytes_per_launch") if d.get("kernel") == kernel else None
 
 
+def batch_summary(steps=20, steps64=4):
+    """Ragged batch: 64 solves (500-200k triangles, p 1..4, half advection
+    and half Euler vortices) in one persistent launch per RK stage, against
+    the same solves one after another on the single-solve path."""
+    import dgcuda_amd as dg
+    from dgcuda_amd import workloads as W
+
+    cases = W.ragged_batch_c4()
+    out = dict(workload="ragged_batch_64_solves_500_to_200k_p1to4", solves=len(cases),
+               triangles=sum(m.num_elem for m, _, _ in cases))
+    for prec, name, n in ((dg.DG_FP32, "fp32", steps), (dg.DG_FP64, "fp64", steps64)):
+        el, work, b = W.run_batch(cases, n, prec)
+        b.close()
+        el2, _, ss = W.run_sequential(cases, n, prec)
+        for x in ss:
+            x.close()
+        out[name] = dict(steps=n, batched_dof_stage_per_s=float("%.4g" % (work / el)),
+                         one_at_a_time_dof_stage_per_s=float("%.4g" % (work / el2)),
+                         speedup=round(el2 / el, 3))
+    return out
+
+
+def serving_summary():
+    """Served (continuous batching) against plain one-at-a-time solves on the
+    same seeded Poisson request streams: the 1k-500k-triangle mix and the
+    small-solve mix."""
+    from dgcuda_amd import workloads as W
+
+    out = {}
+    for name, reqs in (("stream_1k_to_500k_p1to3_32req_40per_s", W.serving_c5()),
+                       ("stream_1k_to_20k_p1to3_128req_1000per_s", W.serving_small())):
+        served, _ = W.run_served(reqs)
+        plain, _ = W.run_plain(reqs)
+        r = lambda d: {k: (float("%.4g" % v) if isinstance(v, float) else v) for k, v in d.items()}
+        out[name] = dict(served=r(served), plain=r(plain),
+                         solves_per_s_ratio=round(served["solves_per_s"] / plain["solves_per_s"], 3))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +135,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the ragged-batch measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving measurements")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run this many steps (for rocprof) and print nothing else")
     args = ap.parse_args()
@@ -160,6 +204,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.p, args.seed)
+    kernel_kind = s.kernel()
+    s.close()  # free the workload before the batch and serving runs
+    batch = None if args.no_batch else batch_summary()
+    serving = None if args.no_serving else serving_summary()
     bw_ms, bw = dg.bandwidth_probe(1 << 30, 10)
     out = {
         "metric": "Euler p=4 DOF*stage updates/s",
@@ -176,7 +224,7 @@ def main():
         "data": "synthetic: seeded 3-vortex isentropic superposition on a seeded jittered periodic mesh",
         "config": {"workload": "euler_rk4_p%d_3vortex_%dx%d_periodic" % (args.p, args.n, args.n),
                    "triangles": K, "p": args.p, "n_p": n_p, "dof": dof, "cfl": 0.3, "integrator": "rk4",
-                   "kernel": s.kernel(), "geometry": "fp64-computed"},
+                   "kernel": kernel_kind, "geometry": "fp64-computed"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(bw, 1),
@@ -185,6 +233,10 @@ def main():
         "t_reached": t_reached,
         "host": platform.node(),
     }
+    if batch is not None:
+        out["batch"] = batch
+    if serving is not None:
+        out["serving"] = serving
     out.update(extra)
     print(json.dumps(out), flush=True)
 

@@ -58,7 +58,8 @@ class AuditOut(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("max_slots", C.c_int), ("hbm_budget", C.c_int64),
-                ("steps_per_round", C.c_int), ("cfl", C.c_double), ("use_graph", C.c_int)]
+                ("steps_per_round", C.c_int), ("cfl", C.c_double), ("use_graph", C.c_int),
+                ("solo_min_elems", C.c_int), ("prep_threads", C.c_int)]
 
 
 class Request(C.Structure):
@@ -400,12 +401,18 @@ class Batch:
         self.close()
 
 
+# A solve of this many triangles fills the GPU by itself (about one tile per
+# resident workgroup): larger solves run their own kernels inside the step.
+SOLO_MIN_ELEMS = 65536
+
+
 class Server:
     """Continuous-batching solve server."""
 
     def __init__(self, precision=DG_FP32, max_slots=64, hbm_budget=64 << 30, steps_per_round=8, cfl=0.3,
-                 use_graph=True):
-        cfg = ServerConfig(precision, max_slots, hbm_budget, steps_per_round, cfl, 1 if use_graph else 0)
+                 use_graph=True, solo_min_elems=SOLO_MIN_ELEMS, prep_threads=4):
+        cfg = ServerConfig(precision, max_slots, hbm_budget, steps_per_round, cfl, 1 if use_graph else 0,
+                           solo_min_elems, prep_threads)
         self._h = C.c_void_p()
         _check(lib().dg_server_create(C.byref(cfg), C.byref(self._h)))
 

@@ -13,6 +13,10 @@ class BatchEngine {
     virtual ~BatchEngine() {}
     // uploads the slot table and work list of these solves (stream-ordered)
     virtual Status set_slots(const std::vector<Solver*>& solvers) = 0;
+    // Solves with at least this many elements (0: none) keep their slot
+    // (per-solve dt and status) but run their own single-solve stage
+    // kernels inside the captured step, after the batch launch.
+    virtual void set_solo_threshold(int min_elems) = 0;
     // enqueues one step (per-solve dt + 4 stages) for every resident solve
     virtual Status step(bool use_graph) = 0;
     // per slot: done flag, steps, t (waits for the engine's stream)

@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(256, 2) k_batch_stage_f64(const SlotDesc<double>* __restrict__ slots,
+__global__ void __launch_bounds__(256, 1) k_batch_stage_f64(const SlotDesc<double>* __restrict__ slots,
                                                            const int2* __restrict__ items,
                                                            const int* __restrict__ parts) {
     __shared__ double sh[8];
@@ -286,6 +286,8 @@ class EngineT : public BatchEngine {
     int nslots = 0, nitems = 0, grid = 0, dt_grid = 0;
     size_t lds = 0;
     hipGraphExec_t gexec = nullptr;
+    int solo_min = 0;             // 0: every solve in the batch launch
+    std::vector<Solver*> solo;    // solves launched with their own kernels
 
     ~EngineT() override {
         if (gexec) hipGraphExecDestroy(gexec);
@@ -340,8 +342,14 @@ class EngineT : public BatchEngine {
         }
     }
 
+    void set_solo_threshold(int min_elems) override {
+        if (min_elems != solo_min) drop_graph();
+        solo_min = std::max(0, min_elems);
+    }
+
     Status set_slots(const std::vector<Solver*>& solvers) override {
         const int ns = (int)solvers.size();
+        std::vector<Solver*> new_solo;
         std::vector<SlotDesc<R>> hs(ns);
         std::vector<int2> lane, tile;
         std::vector<double> lane_cost, tile_cost;
@@ -370,6 +378,10 @@ class EngineT : public BatchEngine {
             d.hcap = si.hcap;
             d.path = tiles ? 2 : 1;
             d.phys = si.phys;
+            if (solo_min > 0 && si.K >= solo_min) {
+                new_solo.push_back(solvers[s]);  // slot kept for dt and status; no work items
+                continue;
+            }
             const int te = tiles ? MFMA_TILE : BATCH_VALU_TILE;
             const int nt = (si.K + te - 1) / te;
             const double per_elem = (double)si.N * n_modes(si.p) * sizeof(R);
@@ -408,6 +420,10 @@ class EngineT : public BatchEngine {
         HIPST(hipStreamSynchronize(st));  // the host vectors go out of scope
         nslots = ns;
         nitems = ni;
+        if (new_solo != solo) {
+            solo = new_solo;
+            drop_graph();  // the solo launches are part of the captured step
+        }
         return Status::ok();
     }
 
@@ -427,8 +443,11 @@ class EngineT : public BatchEngine {
         hipLaunchKernelGGL((k_batch_dt<R>), dim3(dt_grid), dim3(64), 0, st, d_slots, d_xcd + BATCH_NSLOT_AT);
         HIPST(hipGetLastError());
         Status s0;
-        for (int k : {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4})
+        for (int k : {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4}) {
             if (!(s0 = launch_stage(k))) return s0;
+            for (Solver* sv : solo)
+                if (!(s0 = sv->stage_on(k, st))) return s0;
+        }
         return Status::ok();
     }
 
@@ -494,13 +513,10 @@ Status BatchEngine::create(int precision, int max_slots, int max_items, BatchEng
 
 Status make_batch_solver(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out) {
     if (p < 1 || p > 4) return Status::err(DG_ERR_UNSUPPORTED, "batched solves support p = 1..4");
+    // the batch's fixed per-order choice (no autotuning launches at setup)
     Solver* s = nullptr;
-    Status st = Solver::create(m, prob, p, precision, &s);
+    Status st = Solver::create(m, prob, p, precision, &s, (precision == DG_FP32 && p >= 3) ? 2 : 1);
     if (!st) return st;
-    if (precision == DG_FP32 && !(st = s->set_kernel(p >= 3 ? 2 : 1))) {
-        delete s;
-        return st;
-    }
     *out = s;
     return Status::ok();
 }

@@ -1,7 +1,7 @@
 // Continuous-batching solve server.
 //
-// Requests carry an arrival time on the server clock.  A preparation thread
-// takes arrived requests in arrival order while a resident slot and device
+// Requests carry an arrival time on the server clock.  Preparation threads
+// take arrived requests in arrival order while a resident slot and device
 // memory under the HBM budget are free, builds the mesh, sets up the solve
 // on the device (geometry, tiling, initial condition, FP64 audit of the
 // initial state) on the solve's own stream, and hands it to the GPU thread.
@@ -14,6 +14,7 @@
 //
 // A solve's arithmetic depends only on its own data (per-solve dt, no
 // cross-solve reductions), so its result is the same alone and in any mix.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -70,7 +71,8 @@ class ServerImpl : public Server {
     uint64_t next_id = 1, n_submitted = 0, n_finished = 0;
     bool stop = false;
     std::string fatal;
-    std::thread prep_thread, gpu_thread;
+    std::vector<std::thread> prep_threads;
+    std::thread gpu_thread;
 
     double now() override {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -82,7 +84,8 @@ class ServerImpl : public Server {
             stop = true;
         }
         cv.notify_all();
-        if (prep_thread.joinable()) prep_thread.join();
+        for (std::thread& th : prep_threads)
+            if (th.joinable()) th.join();
         if (gpu_thread.joinable()) gpu_thread.join();
         for (Job& j : ready) delete j.s;
         for (Job& j : active) delete j.s;
@@ -93,7 +96,8 @@ class ServerImpl : public Server {
         t0 = std::chrono::steady_clock::now();
         Status st = BatchEngine::create(cfg.precision, cfg.max_slots, 1 << 16, &eng);
         if (!st) return st;
-        prep_thread = std::thread([this] { prep_loop(); });
+        eng->set_solo_threshold(cfg.solo_min_elems);
+        for (int i = 0; i < std::max(1, cfg.prep_threads); i++) prep_threads.emplace_back([this] { prep_loop(); });
         gpu_thread = std::thread([this] { gpu_loop(); });
         return Status::ok();
     }
@@ -317,7 +321,9 @@ class ServerImpl : public Server {
 Status Server::create(const dg_server_config& cfg, Server** out) {
     if (cfg.precision != DG_FP32 && cfg.precision != DG_FP64) return Status::err(DG_ERR_ARG, "bad precision");
     if (cfg.max_slots < 1 || cfg.max_slots > 4096) return Status::err(DG_ERR_ARG, "max_slots must be in 1..4096");
-    if (cfg.hbm_budget <= 0 || cfg.steps_per_round < 1 || !(cfg.cfl > 0)) return Status::err(DG_ERR_ARG, "bad config");
+    if (cfg.hbm_budget <= 0 || cfg.steps_per_round < 1 || !(cfg.cfl > 0) || cfg.solo_min_elems < 0 ||
+        cfg.prep_threads < 0 || cfg.prep_threads > 64)
+        return Status::err(DG_ERR_ARG, "bad config");
     auto* s = new ServerImpl();
     s->cfg = cfg;
     Status st = s->start();

@@ -67,7 +67,12 @@ class Solver {
     // the last stage of a step issued elsewhere left the wave speed in ctrl
     virtual void set_lambda_valid(bool v) = 0;
     virtual hipStream_t get_stream() const = 0;
-    static Status create(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out);
+    // enqueue RK4 stage `kind` (ST_RK4_1..4) of the current step with this
+    // solve's own kernel on stream `st` (its dt is already in the control block)
+    virtual Status stage_on(int kind, hipStream_t st) = 0;
+    // kernel: 0 = automatic (timed by the autotuner), 1..3 = that kind
+    static Status create(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out,
+                         int kernel = 0);
 
     int K = 0, S = 0, Kp = 0, p = 0, NP = 0, NQ = 0, NQ1 = 0, N = 0, precision = DG_FP64;
     double min_r = 0;

@@ -147,7 +147,7 @@ class SolverT : public Solver {
         return Status::ok();
     }
 
-    Status init(const Mesh& m, const dg_problem_desc& pr, int p_) {
+    Status init(const Mesh& m, const dg_problem_desc& pr, int p_, int kernel) {
         prob = pr;
         p = p_;
         N = pr.physics;
@@ -222,7 +222,7 @@ class SolverT : public Solver {
         hctrl = Ctrl{};
         hctrl.min_r = min_r;
         hctrl.two_p1 = 2. * p + 1.;
-        if (!(st = choose_shape(0))) return st;
+        if (!(st = choose_shape(kernel))) return st;
         if (!(st = push_ctrl())) return st;
         HIPST(hipStreamSynchronize(stream));
         return Status::ok();
@@ -396,7 +396,7 @@ class SolverT : public Solver {
         return Status::ok();
     }
 
-    Status stage(int kind, const R* u, const R* cc, R* out) {
+    Status stage(int kind, const R* u, const R* cc, R* out, hipStream_t on = nullptr) {
         StageArgs<R> a{};
         a.u = u;
         a.c = cc;
@@ -414,15 +414,26 @@ class SolverT : public Solver {
         a.xcd_remap = shape.xcd_remap;
         a.phys = phys;
         hipError_t e;
+        hipStream_t sx = on ? on : stream;
         if constexpr (sizeof(R) == 4) {
             if (shape.kind >= 2)
-                e = launch_stage_mfma(N, p, a, mx, dim3(shape.grid), stream);
+                e = launch_stage_mfma(N, p, a, mx, dim3(shape.grid), sx);
             else
-                e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), stream);
+                e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), sx);
         } else {
-            e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), stream);
+            e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), sx);
         }
         return hip_status(e, "stage kernel launch");
+    }
+
+    Status stage_on(int kind, hipStream_t st) override {
+        switch (kind) {
+            case ST_RK4_1: return stage(ST_RK4_1, c, c, ua, st);
+            case ST_RK4_2: return stage(ST_RK4_2, ua, c, ub, st);
+            case ST_RK4_3: return stage(ST_RK4_3, ub, c, ua, st);
+            case ST_RK4_4: return stage(ST_RK4_4, ua, c, c, st);
+            default: return Status::err(DG_ERR_ARG, "stage_on: RK4 stages only");
+        }
     }
 
     Status ensure_lambda() {
@@ -743,12 +754,12 @@ class SolverT : public Solver {
     }
 };
 
-Status Solver::create(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out) {
+Status Solver::create(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out, int kernel) {
     Status st = check_problem(prob, p, precision);
     if (!st) return st;
     if (precision == DG_FP64) {
         auto* s = new SolverT<double>();
-        st = s->init(m, prob, p);
+        st = s->init(m, prob, p, kernel);
         if (!st) {
             delete s;
             return st;
@@ -756,7 +767,7 @@ Status Solver::create(const Mesh& m, const dg_problem_desc& prob, int p, int pre
         *out = s;
     } else {
         auto* s = new SolverT<float>();
-        st = s->init(m, prob, p);
+        st = s->init(m, prob, p, kernel);
         if (!st) {
             delete s;
             return st;

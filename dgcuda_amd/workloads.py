@@ -8,6 +8,7 @@ measurements built on them.
                     half advection (Gaussian pulses), half Euler vortices
   serving_c5()      Poisson request stream, log-uniform 1k..500k triangles,
                     p in 1..3, t_end in [0.5, 2]
+  serving_small()   the same stream with 1k..20k triangles
 """
 import math
 import time
@@ -75,6 +76,12 @@ def serving_c5(seed=11, n=32, rate=40.0, kmin=1000, kmax=500_000):
         prob = euler_vortex(((float(rng.uniform(2, 5)), float(rng.uniform(0, 10)), float(rng.uniform(0, 10))),))
         reqs.append(request(nx, p, te, prob, mesh_seed=int(rng.integers(1, 2**31)), arrival_s=t))
     return reqs
+
+
+def serving_small(seed=13, n=128, rate=1000.0, kmin=1000, kmax=20_000):
+    """The serving stream restricted to small solves (1k-20k triangles):
+    the regime where one-at-a-time solving is bound by launch latency."""
+    return serving_c5(seed=seed, n=n, rate=rate, kmin=kmin, kmax=kmax)
 
 
 def dof_stage_per_step(mesh, prob, p):

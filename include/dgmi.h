@@ -210,6 +210,9 @@ typedef struct {
     int steps_per_round;    /* batch steps between admissions / completions */
     double cfl;
     int use_graph;
+    int solo_min_elems;     /* solves with at least this many elements run their own stage
+                               kernels inside the step graph (0: all in the batch launch) */
+    int prep_threads;       /* host threads preparing admitted solves (mesh, setup, projection) */
 } dg_server_config;
 
 typedef struct {

@@ -27,18 +27,28 @@ def requests(n, seed=3):
     return out
 
 
-def alone(req):
-    m = dg.Mesh.periodic_jittered(req.nx, req.ny, req.x0, req.x1, req.y0, req.y1, req.jitter, req.mesh_seed)
-    b = dg.Batch(dg.DG_FP32)
-    b.add(m, req.prob, req.p, req.t_end)
-    b.project_ic()
-    b.integrate(0.3)
-    return b.get_state(0)
+def alone(req, solo_min_elems):
+    """The same request served by itself."""
+    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems)
+    sv.keep_states(True)
+    r = dg.request(req.nx, req.p, req.t_end, req.prob, mesh_seed=req.mesh_seed, arrival_s=0.0)
+    i = sv.submit(r)
+    sv.drain()
+    out = sv.poll(4)
+    assert len(out) == 1 and out[0].status == 0
+    state = sv.get_state(i, (4, dg.n_modes(req.p), 2 * req.nx * req.ny))
+    sv.close()
+    return state
 
 
-def test_server_results_equal_alone():
+@pytest.mark.parametrize("solo_min_elems", [0, 400])
+def test_server_results_equal_alone(solo_min_elems):
+    """In the mix, every result equals the request served alone; with a
+    threshold of 400 elements the larger requests run their own kernels
+    inside the step graph while the others share the batch launch."""
     reqs = requests(10)
-    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3)
+    assert any(2 * r.nx * r.ny >= 400 for r in reqs) and any(2 * r.nx * r.ny < 400 for r in reqs)
+    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems)
     sv.keep_states(True)
     t0 = sv.now()
     ids = []
@@ -48,13 +58,13 @@ def test_server_results_equal_alone():
     sv.drain()
     res = {r.id: r for r in sv.poll(64)}
     assert len(res) == len(reqs)
-    for k in (0, 4, 9):
+    for k in range(len(reqs)):
         r = reqs[k]
         out = res[ids[k]]
         assert out.status == 0 and abs(out.t - r.t_end) == 0.0
         shape = (4, dg.n_modes(r.p), 2 * r.nx * r.ny)
         got = sv.get_state(ids[k], shape)
-        assert np.array_equal(got, alone(r)), k
+        assert np.array_equal(got, alone(r, solo_min_elems)), k
     for out in res.values():
         assert out.finish_s >= out.arrival_s
         m0, m1 = np.array(out.mass0), np.array(out.mass)
