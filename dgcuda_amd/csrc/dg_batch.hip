@@ -290,6 +290,7 @@ class EngineT : public BatchEngine {
     std::vector<Solver*> solo;    // solves launched with their own kernels
 
     ~EngineT() override {
+        if (st) (void)hipStreamSynchronize(st);
         if (gexec) hipGraphExecDestroy(gexec);
         void* b[] = {d_slots, d_items, d_xcd, d_status};
         for (void* q : b)

@@ -30,6 +30,17 @@ struct Status {
 Status hip_status(hipError_t e, const char* what);
 Status ensure_device_ready();
 
+// Device memory and streams for solves, kept for reuse within the process:
+// hipFree and hipStreamDestroy synchronise the whole device, which a serving
+// loop that retires solves while others run cannot afford.  Blocks are
+// reused best-fit (at most 1.5x the request); at most 16 GiB of idle blocks
+// are kept per device.  Callers return memory only when no queued work uses
+// it (each solver synchronises its own stream first).
+Status pool_alloc(void** p, size_t bytes);
+void pool_free(void* p);
+Status pool_stream(hipStream_t* s);
+void pool_release_stream(hipStream_t s);
+
 PhysP make_phys(const dg_problem_desc& d);
 IcP make_ic(const dg_problem_desc& d, double Lx, double Ly);
 Status check_problem(const dg_problem_desc& d, int p, int precision);

@@ -46,6 +46,100 @@ Status ensure_device_ready() {
     return Status::ok();
 }
 
+namespace {
+struct Pool {
+    std::mutex mu;
+    std::map<void*, std::pair<int, size_t>> live;           // block -> (device, bytes)
+    std::multimap<std::pair<int, size_t>, void*> idle;      // (device, bytes) -> block
+    std::map<int, size_t> idle_bytes;
+    std::map<int, std::vector<hipStream_t>> streams;
+};
+Pool& pool() {
+    static Pool* p = new Pool();  // never destroyed: blocks live until the process ends
+    return *p;
+}
+constexpr size_t POOL_IDLE_CAP = size_t(16) << 30;
+size_t pool_round(size_t n) {
+    const size_t g = n <= (size_t(1) << 20) ? 4096 : (size_t(2) << 20);
+    return (n + g - 1) / g * g;
+}
+}  // namespace
+
+Status pool_alloc(void** out, size_t bytes) {
+    int dev = 0;
+    HIPST(hipGetDevice(&dev));
+    const size_t need = pool_round(std::max<size_t>(bytes, 1));
+    Pool& P = pool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto it = P.idle.lower_bound(std::make_pair(dev, need));
+        if (it != P.idle.end() && it->first.first == dev && it->first.second <= need + need / 2) {
+            *out = it->second;
+            P.live[it->second] = it->first;
+            P.idle_bytes[dev] -= it->first.second;
+            P.idle.erase(it);
+            return Status::ok();
+        }
+    }
+    HIPST(hipMalloc(out, need));
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.live[*out] = std::make_pair(dev, need);
+    return Status::ok();
+}
+
+void pool_free(void* p) {
+    if (!p) return;
+    Pool& P = pool();
+    std::vector<void*> release;
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto it = P.live.find(p);
+        if (it == P.live.end()) return;
+        const auto key = it->second;
+        P.live.erase(it);
+        P.idle.emplace(key, p);
+        size_t& ib = P.idle_bytes[key.first];
+        ib += key.second;
+        // over the cap: give back the largest idle blocks of this device
+        while (ib > POOL_IDLE_CAP) {
+            auto last = P.idle.lower_bound(std::make_pair(key.first + 1, size_t(0)));
+            if (last == P.idle.begin()) break;
+            --last;
+            if (last->first.first != key.first) break;
+            ib -= last->first.second;
+            release.push_back(last->second);
+            P.idle.erase(last);
+        }
+    }
+    for (void* q : release) (void)hipFree(q);
+}
+
+Status pool_stream(hipStream_t* s) {
+    int dev = 0;
+    HIPST(hipGetDevice(&dev));
+    Pool& P = pool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto& v = P.streams[dev];
+        if (!v.empty()) {
+            *s = v.back();
+            v.pop_back();
+            return Status::ok();
+        }
+    }
+    HIPST(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
+    return Status::ok();
+}
+
+void pool_release_stream(hipStream_t s) {
+    if (!s) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.streams[dev].push_back(s);
+}
+
 PhysP make_phys(const dg_problem_desc& d) {
     PhysP ph{};
     ph.gamma = d.gamma;
@@ -132,17 +226,18 @@ class SolverT : public Solver {
     double b0 = 0;  // phi_0 at the first volume point (cell mean = c_0 * b0)
 
     ~SolverT() override {
-        if (gexec) hipGraphExecDestroy(gexec);
+        if (stream) (void)hipStreamSynchronize(stream);  // nothing queued may use the buffers
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, audit_part, ctrl, fword, halo};
-        for (void* b : bufs)
-            if (b) hipFree(b);
-        if (stream) hipStreamDestroy(stream);
+        for (void* b : bufs) pool_free(b);
+        pool_release_stream(stream);
     }
 
     size_t state_elems() const { return (size_t)N * NP * Kp; }
 
     Status alloc(void** p, size_t n) {
-        HIPST(hipMalloc(p, n));
+        Status st = pool_alloc(p, n);
+        if (!st) return st;
         bytes += (int64_t)n;
         return Status::ok();
     }
@@ -175,7 +270,7 @@ class SolverT : public Solver {
         min_r = g.min_r;
         Status st = ensure_device_ready();
         if (!st) return st;
-        HIPST(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (!(st = pool_stream(&stream))) return st;
         const size_t M = state_elems();
         if (!(st = alloc((void**)&c, M * sizeof(R)))) return st;
         if (!(st = alloc((void**)&acc, M * sizeof(R)))) return st;
@@ -262,12 +357,14 @@ class SolverT : public Solver {
         for (int t = 0; t < ntiles; t++)
             for (int i = 0; i < hcap; i++)
                 hl[(size_t)t * hcap + i] = i < (int)halos[t].size() ? halos[t][i] : t * tile_elems;
-        if (fword) hipFree(fword);
-        if (halo) hipFree(halo);
+        HIPST(hipStreamSynchronize(stream));  // queued launches may still read the old lists
+        pool_free(fword);
+        pool_free(halo);
         fword = nullptr;
         halo = nullptr;
-        HIPST(hipMalloc((void**)&fword, words.size() * sizeof(uint16_t)));
-        HIPST(hipMalloc((void**)&halo, hl.size() * sizeof(int)));
+        Status sa = pool_alloc((void**)&fword, words.size() * sizeof(uint16_t));
+        if (sa) sa = pool_alloc((void**)&halo, hl.size() * sizeof(int));
+        if (!sa) return sa;
         HIPST(hipMemcpyAsync(fword, words.data(), words.size() * sizeof(uint16_t), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(halo, hl.data(), hl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
         HIPST(hipStreamSynchronize(stream));
