@@ -69,7 +69,7 @@ __device__ __forceinline__ int batch_mfma_run(const StageArgs<float>& a, const M
     A.init(g, j, it >> 31);
     MfmaTileState<N, P> cur, nxt;
     int tile = uni(items[it].y);
-    cur.load(a.u, a.K, a.Kp, mx.halo, mx.hcap, tile, w, g, j);
+    cur.load(a.u, a.K, mx.halo, mx.hcap, tile, w, g, j);
     while (true) {
         const int nit = it + G8;
         int ntile = -1;
@@ -77,7 +77,7 @@ __device__ __forceinline__ int batch_mfma_run(const StageArgs<float>& a, const M
             const int2 ni = items[nit];
             if (uni(ni.x) == slot) ntile = uni(ni.y);
         }
-        if (ntile >= 0) nxt.load(a.u, a.K, a.Kp, mx.halo, mx.hcap, ntile, w, g, j);
+        if (ntile >= 0) nxt.load(a.u, a.K, mx.halo, mx.hcap, ntile, w, g, j);
         lam = fmax(lam, mfma_process_tile<N, P, KIND>(a, mx, A, cur, tile, fd, ts));
         if (ntile < 0) return nit;
         it = nit;

@@ -149,9 +149,33 @@ enum StageKind {
     ST_RK2_2 = 6, // c = c + k; wave-speed partials
 };
 
+// Placement of entry (variable v, mode m, element e) of a state array in HBM.
+//  * Mode-major (the lane-per-element kernels): (v*NP + m)*Kp + e, so the
+//    lanes of a wavefront read consecutive elements of one mode.
+//  * Tile layout (FP32 state of the matrix-core kernels): per 16-element
+//    block and variable, G = ceil(NP/4) groups of [16 elements][4 modes],
+//    (((e/16)*N + v)*G + m/4)*64 + (e%16)*4 + m%4.  Lane j + 16g of a
+//    wavefront holds modes 4g..4g+3 of element j, one 16-byte access, and
+//    the wavefront's 64 lanes cover 1 KiB contiguous.  The modes n_p..4G-1
+//    of the last group are padding and hold zeros.
+constexpr int tile_groups(int NP) { return (NP + 3) / 4; }
+
+struct StateIx {
+    int N, NP, Kp, tiled;
+    __host__ __device__ size_t at(int v, int m, int e) const {
+        if (!tiled) return ((size_t)v * NP + m) * Kp + e;
+        const int G = tile_groups(NP);
+        return ((((size_t)(e >> 4) * N + v) * G + (m >> 2)) << 6) + (size_t)((e & 15) << 2) + (m & 3);
+    }
+    // entries of one array: N*NP*Kp, or N*4G*Kp with the padding modes
+    __host__ __device__ size_t elems() const {
+        return (size_t)N * (tiled ? 4 * tile_groups(NP) : NP) * Kp;
+    }
+};
+
 template <typename R>
 struct StageArgs {
-    const R* u;       // stage input [N][n_p][Kp]
+    const R* u;       // stage input [N][n_p][Kp] in the kernel's state layout (StateIx)
     const R* c;       // RK base state
     R* acc;           // RK accumulator
     R* out;           // next stage input, k, or c (in place) for the last stage
@@ -160,7 +184,7 @@ struct StageArgs {
     const int* nbr;   // [3][Kp]
     const Ctrl* ctrl;
     unsigned long long* lam_bits; // &ctrl->lam_bits (last stage)
-    int K, Kp;
+    int K, Kp;        // elements, and the padded count (multiple of 64) of every [..][Kp] array
     int kind;
     int ntiles;       // tiles (workgroups) that carry elements
     int xcd_remap;    // map consecutive tiles to one XCD

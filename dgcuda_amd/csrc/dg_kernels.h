@@ -36,9 +36,9 @@ struct MfmaArgs {
     int pad;
 };
 
-// matrix-core element kernel (FP32, p = 3, 4): 64-element tiles, 4
-// wavefronts per workgroup, persistent over the tiles of its XCD's range
-// when the grid is smaller than the tile count; returns
+// matrix-core element kernel (FP32, p = 1..4, state in the tile layout):
+// 64-element tiles, 4 wavefronts per workgroup, persistent over the tiles
+// of its XCD's range when the grid is smaller than the tile count; returns
 // hipErrorInvalidValue for unsupported (N, p)
 constexpr int MFMA_TILE_ELEMS = 64;
 constexpr int MFMA_MAX_HALO = 64;
@@ -51,14 +51,14 @@ constexpr int VALU_TILE_ELEMS = 128;
 
 // resets ctrl->lam_bits and accumulates the state's maximum cell wave speed into it
 template <typename R>
-hipError_t launch_lambda(int N, const R* u, int K, int Kp, int NP, double b0, const PhysP& ph, Ctrl* ctrl,
+hipError_t launch_lambda(const R* u, const StateIx& ix, int K, double b0, const PhysP& ph, Ctrl* ctrl,
                          hipStream_t st);
 hipError_t launch_dt(Ctrl* ctrl, hipStream_t st);
 hipError_t launch_set_dt(Ctrl* ctrl, double dt, int advance, hipStream_t st);
 template <typename R>
-hipError_t launch_project_ic(int N, int p, R* c, const double* V, int K, int Kp, const IcP& ic, hipStream_t st);
+hipError_t launch_project_ic(int p, R* c, const StateIx& ix, const double* V, int K, const IcP& ic, hipStream_t st);
 template <typename R>
-hipError_t launch_audit(int N, const R* c, const double* Jd, int K, int Kp, int NP, double b0, double gamma,
+hipError_t launch_audit(const R* c, const StateIx& ix, const double* Jd, int K, double b0, double gamma,
                         double* part, int nblocks, hipStream_t st);
 
 }  // namespace dgmi

@@ -38,14 +38,14 @@ __global__ void __launch_bounds__(128) k_stage_valu(StageArgs<R> a) {
 // Maximum cell wave speed of a state without stepping it (first step of a
 // run, dg_cfl_dt): grid-stride over elements, workgroup max, one atomic max.
 template <int N, typename R>
-__global__ void __launch_bounds__(256) k_lambda(const R* u, int K, int Kp, int NP, double b0, PhysP ph,
+__global__ void __launch_bounds__(256) k_lambda(const R* u, StateIx ix, int K, double b0, PhysP ph,
                                                unsigned long long* lam_bits) {
     __shared__ double sh[8];
     double mx = 0.0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K; e += gridDim.x * blockDim.x) {
         double U[4];
 #pragma unroll
-        for (int v = 0; v < N; v++) U[v] = (double)u[(size_t)(v * NP) * Kp + e] * b0;
+        for (int v = 0; v < N; v++) U[v] = (double)u[ix.at(v, 0, e)] * b0;
         mx = fmax(mx, cell_speed<N>(ph, U));
     }
     mx = block_max(mx, sh);
@@ -115,8 +115,9 @@ __device__ void ic_point(const IcP& ic, double x, double y, double* U) {
 }
 
 template <int N, int P, typename R>
-__global__ void __launch_bounds__(128) k_project_ic(R* c, const double* V, int K, int Kp, IcP ic) {
+__global__ void __launch_bounds__(128) k_project_ic(R* c, StateIx ix, const double* V, int K, IcP ic) {
     constexpr int NP = n_modes(P), NQ = n_quad_vol(P);
+    const int Kp = ix.Kp;
     const TabP<double, P>& T = tab<double, P>();
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= K) return;
@@ -136,8 +137,11 @@ __global__ void __launch_bounds__(128) k_project_ic(R* c, const double* V, int K
         for (int n = 0; n < N; n++) U[n] = 0.;
         for (int j = 0; j < NQ; j++)
             for (int n = 0; n < N; n++) U[n] += T.W[j] * u0[j][n] * T.B[i][j];
-        for (int n = 0; n < N; n++) c[(size_t)(n * NP + i) * Kp + e] = (R)U[n];
+        for (int n = 0; n < N; n++) c[ix.at(n, i, e)] = (R)U[n];
     }
+    if (ix.tiled)  // padding modes of the last group
+        for (int i = NP; i < 4 * tile_groups(NP); i++)
+            for (int n = 0; n < N; n++) c[ix.at(n, i, e)] = R(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -146,7 +150,7 @@ __global__ void __launch_bounds__(128) k_project_ic(R* c, const double* V, int K
 // part[blk*8 + 0..3] mass, +4 min rho, +5 min p, +6 bad count.
 // ---------------------------------------------------------------------------
 template <int N, typename R>
-__global__ void __launch_bounds__(256) k_audit(const R* c, const double* Jd, int K, int Kp, int NP, double b0,
+__global__ void __launch_bounds__(256) k_audit(const R* c, StateIx ix, const double* Jd, int K, double b0,
                                               double gamma, double* part) {
     __shared__ double sm[256];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,7 +159,7 @@ __global__ void __launch_bounds__(256) k_audit(const R* c, const double* Jd, int
         double U[4];
         const double J = Jd[e];
         for (int v = 0; v < N; v++) {
-            double c0 = (double)c[(size_t)(v * NP) * Kp + e];
+            double c0 = (double)c[ix.at(v, 0, e)];
             mass[v] = J * c0 * b0 * 0.5;
             U[v] = c0 * b0;
         }
@@ -212,19 +216,20 @@ template hipError_t launch_stage_valu<float>(int, int, const StageArgs<float>&, 
 template hipError_t launch_stage_valu<double>(int, int, const StageArgs<double>&, dim3, hipStream_t);
 
 template <typename R>
-hipError_t launch_lambda(int N, const R* u, int K, int Kp, int NP, double b0, const PhysP& ph, Ctrl* ctrl,
+hipError_t launch_lambda(const R* u, const StateIx& ix, int K, double b0, const PhysP& ph, Ctrl* ctrl,
                          hipStream_t st) {
     hipLaunchKernelGGL(k_reset_lam, dim3(1), dim3(64), 0, st, ctrl);
     const dim3 grid(std::min((K + 255) / 256, 2048));
     unsigned long long* bits = &ctrl->lam_bits;
-    if (N == 1)
-        hipLaunchKernelGGL((k_lambda<1, R>), grid, dim3(256), 0, st, u, K, Kp, NP, b0, ph, bits);
+    if (ix.N == 1)
+        hipLaunchKernelGGL((k_lambda<1, R>), grid, dim3(256), 0, st, u, ix, K, b0, ph, bits);
     else
-        hipLaunchKernelGGL((k_lambda<4, R>), grid, dim3(256), 0, st, u, K, Kp, NP, b0, ph, bits);
+        hipLaunchKernelGGL((k_lambda<4, R>), grid, dim3(256), 0, st, u, ix, K, b0, ph, bits);
     return hipGetLastError();
 }
-template hipError_t launch_lambda<float>(int, const float*, int, int, int, double, const PhysP&, Ctrl*, hipStream_t);
-template hipError_t launch_lambda<double>(int, const double*, int, int, int, double, const PhysP&, Ctrl*,
+template hipError_t launch_lambda<float>(const float*, const StateIx&, int, double, const PhysP&, Ctrl*,
+                                         hipStream_t);
+template hipError_t launch_lambda<double>(const double*, const StateIx&, int, double, const PhysP&, Ctrl*,
                                           hipStream_t);
 
 hipError_t launch_dt(Ctrl* ctrl, hipStream_t st) {
@@ -238,41 +243,44 @@ hipError_t launch_set_dt(Ctrl* ctrl, double dt, int advance, hipStream_t st) {
 }
 
 template <int N, typename R>
-static hipError_t launch_ic_np(int p, R* c, const double* V, int K, int Kp, const IcP& ic, hipStream_t st) {
+static hipError_t launch_ic_np(int p, R* c, const StateIx& ix, const double* V, int K, const IcP& ic,
+                               hipStream_t st) {
     dim3 grid((K + 127) / 128);
     switch (p) {
-        case 0: hipLaunchKernelGGL((k_project_ic<N, 0, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
-        case 1: hipLaunchKernelGGL((k_project_ic<N, 1, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
-        case 2: hipLaunchKernelGGL((k_project_ic<N, 2, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
-        case 3: hipLaunchKernelGGL((k_project_ic<N, 3, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
-        case 4: hipLaunchKernelGGL((k_project_ic<N, 4, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
-        case 5: hipLaunchKernelGGL((k_project_ic<N, 5, R>), grid, dim3(128), 0, st, c, V, K, Kp, ic); break;
+        case 0: hipLaunchKernelGGL((k_project_ic<N, 0, R>), grid, dim3(128), 0, st, c, ix, V, K, ic); break;
+        case 1: hipLaunchKernelGGL((k_project_ic<N, 1, R>), grid, dim3(128), 0, st, c, ix, V, K, ic); break;
+        case 2: hipLaunchKernelGGL((k_project_ic<N, 2, R>), grid, dim3(128), 0, st, c, ix, V, K, ic); break;
+        case 3: hipLaunchKernelGGL((k_project_ic<N, 3, R>), grid, dim3(128), 0, st, c, ix, V, K, ic); break;
+        case 4: hipLaunchKernelGGL((k_project_ic<N, 4, R>), grid, dim3(128), 0, st, c, ix, V, K, ic); break;
+        case 5: hipLaunchKernelGGL((k_project_ic<N, 5, R>), grid, dim3(128), 0, st, c, ix, V, K, ic); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_project_ic(int N, int p, R* c, const double* V, int K, int Kp, const IcP& ic, hipStream_t st) {
-    if (N == 1) return launch_ic_np<1, R>(p, c, V, K, Kp, ic, st);
-    return launch_ic_np<4, R>(p, c, V, K, Kp, ic, st);
+hipError_t launch_project_ic(int p, R* c, const StateIx& ix, const double* V, int K, const IcP& ic, hipStream_t st) {
+    if (ix.N == 1) return launch_ic_np<1, R>(p, c, ix, V, K, ic, st);
+    return launch_ic_np<4, R>(p, c, ix, V, K, ic, st);
 }
-template hipError_t launch_project_ic<float>(int, int, float*, const double*, int, int, const IcP&, hipStream_t);
-template hipError_t launch_project_ic<double>(int, int, double*, const double*, int, int, const IcP&, hipStream_t);
+template hipError_t launch_project_ic<float>(int, float*, const StateIx&, const double*, int, const IcP&,
+                                             hipStream_t);
+template hipError_t launch_project_ic<double>(int, double*, const StateIx&, const double*, int, const IcP&,
+                                              hipStream_t);
 
 template <typename R>
-hipError_t launch_audit(int N, const R* c, const double* Jd, int K, int Kp, int NP, double b0, double gamma,
+hipError_t launch_audit(const R* c, const StateIx& ix, const double* Jd, int K, double b0, double gamma,
                         double* part, int nblocks, hipStream_t st) {
-    if (N == 1)
-        hipLaunchKernelGGL((k_audit<1, R>), dim3(nblocks), dim3(256), 0, st, c, Jd, K, Kp, NP, b0, gamma, part);
+    if (ix.N == 1)
+        hipLaunchKernelGGL((k_audit<1, R>), dim3(nblocks), dim3(256), 0, st, c, ix, Jd, K, b0, gamma, part);
     else
-        hipLaunchKernelGGL((k_audit<4, R>), dim3(nblocks), dim3(256), 0, st, c, Jd, K, Kp, NP, b0, gamma, part);
+        hipLaunchKernelGGL((k_audit<4, R>), dim3(nblocks), dim3(256), 0, st, c, ix, Jd, K, b0, gamma, part);
     return hipGetLastError();
 }
-template hipError_t launch_audit<float>(int, const float*, const double*, int, int, int, double, double, double*,
+template hipError_t launch_audit<float>(const float*, const StateIx&, const double*, int, double, double, double*,
                                         int, hipStream_t);
-template hipError_t launch_audit<double>(int, const double*, const double*, int, int, int, double, double, double*,
-                                         int, hipStream_t);
+template hipError_t launch_audit<double>(const double*, const StateIx&, const double*, int, double, double,
+                                         double*, int, hipStream_t);
 
 // Fill the constant-memory tables of every order from the FP64 host tables.
 template <typename R, int P>

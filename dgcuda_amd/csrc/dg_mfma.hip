@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256, DGMI_MFMA_MINW) k_stage_mfma(StageArgs<fl
     int hid[MfmaTileState<N, P>::HPW];
     if (t < tend) {
         MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, t, w, j, hid);
-        st.load_data(a.u, a.K, a.Kp, mx.hcap, t, w, g, j, hid);
+        st.load_data(a.u, a.K, mx.hcap, t, w, g, j, hid);
     }
     if (w == 0) mfma_a_to_lds<N, P>(tabA, lane);
     __syncthreads();

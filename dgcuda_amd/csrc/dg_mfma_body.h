@@ -41,34 +41,34 @@ __host__ __device__ constexpr size_t mfma_tile_lds_bytes(int N, int hcap) {
     return (size_t)MFMA_TILE * 3 * 16 + (size_t)(MFMA_TILE + hcap) * 17 * N * 4;
 }
 
-// Global addresses are split into a wavefront-uniform base (scalar
-// registers) and one per-lane 32-bit unsigned offset.
-template <int N, int P>
-__device__ __forceinline__ void load_state(const float* __restrict__ u, int Kp, unsigned lane_off, int g,
-                                           float (&dst)[N][4]) {
-    constexpr int NP = n_modes(P);
-#pragma unroll
-    for (int v = 0; v < N; v++)
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const DG_GLOBAL float* bp = gp(u) + (size_t)(v * NP + 4 * s) * Kp;  // uniform
-            dst[v][s] = (4 * s + g < NP) ? bp[lane_off] : 0.f;   // lane_off = g*Kp + e
-        }
-}
+// State arrays are in the tile layout (StateIx, tiled): lane j + 16g of a
+// wavefront reads modes 4g..4g+3 of its element as one 16-byte access.
+// Addresses are a wavefront-uniform base (scalar registers) plus one
+// per-lane 32-bit float4 index.
+__device__ __forceinline__ const DG_GLOBAL floatx4* gp4(const float* p) { return (const DG_GLOBAL floatx4*)p; }
+__device__ __forceinline__ DG_GLOBAL floatx4* gp4(float* p) { return (DG_GLOBAL floatx4*)p; }
 
+// float4 stride between the variables of one 16-element block
 template <int P>
-__device__ __forceinline__ void load_state_var(const float* __restrict__ u, int Kp, unsigned lane_off, int g, int v,
-                                               int nrows, float (&dst)[4]) {
-    constexpr int NP = n_modes(P);
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const DG_GLOBAL float* bp = gp(u) + (size_t)(v * NP + 4 * s) * Kp;
-        dst[s] = (4 * s + g < NP) ? bp[lane_off] : 0.f;
-    }
+constexpr int tile_var_stride() { return 16 * tile_groups(n_modes(P)); }
+
+// float4 index of (variable 0, modes 4g..4g+3, element e); variable v adds
+// v * tile_var_stride<P>().  Lanes of groups past the last (p <= 3) read
+// the last group again; their operands meet zero matrix entries and they
+// store nothing.
+template <int N, int P>
+__device__ __forceinline__ unsigned tile_lane4(int e, int g) {
+    constexpr int G = tile_groups(n_modes(P));
+    const int gg = (G == 4) ? g : min(g, G - 1);
+    return (unsigned)((e >> 4) * (N * 16 * G) + gg * 16 + (e & 15));
 }
 
 // per-lane A operands of the five products (lane layout: A[i][k] from lane
-// i + 16k, B[k][j] from lane j + 16k, D[4(l/16)+r][l%16] in register r)
+// i + 16k, B[k][j] from lane j + 16k, D[4(l/16)+r][l%16] in register r).
+// The mode reductions (vol, tr) run mode 4g+s through product s in lane
+// group g, so the B operand of a lane is the lane's own 4 consecutive modes
+// (the state's 16-byte access) and the state in operand registers is
+// already in the output layout of the projections.
 template <int N, int P>
 struct MfmaA {
     float vol[4], tr[4], sp[4], gx[4], gy[4];
@@ -81,7 +81,7 @@ struct MfmaA {
             *reinterpret_cast<const TabP<float, P>*>(reinterpret_cast<const float*>(&tab<float, P>()) + z);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            const int m = 4 * s + g;
+            const int m = 4 * g + s;
             vol[s] = (m < NP && j < NQ) ? T.B[m][j] : 0.f;
             tr[s] = (m < NP && j < NPT) ? T.S[j / NQ1][m][j % NQ1] : 0.f;
             const int q = 4 * g + s;
@@ -114,8 +114,9 @@ __device__ __forceinline__ void mfma_a_to_lds(float4* tabA, int lane) {
 template <int N, int P>
 struct MfmaTileState {
     static constexpr int HPW = (4 * N + 3) / 4;  // most halo pairs per wave (hcap <= 64)
-    float ub[N][4];
-    float hb[HPW][4];
+    static constexpr int VS = tile_var_stride<P>();
+    floatx4 ub[N];
+    floatx4 hb[HPW];
     // Halo element ids of this wave's pairs.  A halo state load depends on
     // its id, and waiting for an id waits for every load issued before it,
     // so the ids are requested ahead of the state loads that use them.
@@ -128,49 +129,28 @@ struct MfmaTileState {
             hid[q] = (pr < npairs) ? gp(halo)[(unsigned)(tile * hcap + (pr / N) * 16 + j)] : 0;
         }
     }
-    __device__ __forceinline__ void load_data(const float* __restrict__ u, int K, int Kp, int hcap, int tile, int w,
-                                              int g, int j, const int (&hid)[HPW]) {
-        load_state<N, P>(u, Kp, (unsigned)(g * Kp + min(tile * MFMA_TILE + w * 16 + j, K - 1)), g, ub);
+    __device__ __forceinline__ void load_data(const float* __restrict__ u, int K, int hcap, int tile, int w, int g,
+                                              int j, const int (&hid)[HPW]) {
+        const unsigned l4 = tile_lane4<N, P>(min(tile * MFMA_TILE + w * 16 + j, K - 1), g);
+#pragma unroll
+        for (int v = 0; v < N; v++) ub[v] = gp4(u)[v * VS + l4];
         const int npairs = (hcap >> 4) * N;
-        constexpr int NP = n_modes(P);
 #pragma unroll
         for (int q = 0; q < HPW; q++) {
             const int pr = w + 4 * q;
-            if (pr < npairs) {
-                load_state_var<P>(u, Kp, (unsigned)(g * Kp + hid[q]), g, pr % N, N * NP, hb[q]);
-            } else {
-#pragma unroll
-                for (int s = 0; s < 4; s++) hb[q][s] = 0.f;
-            }
+            if (pr < npairs)
+                hb[q] = gp4(u)[(pr % N) * VS + tile_lane4<N, P>(hid[q], g)];
+            else
+                hb[q] = floatx4{0.f, 0.f, 0.f, 0.f};
         }
     }
-    __device__ __forceinline__ void load(const float* __restrict__ u, int K, int Kp, const int* __restrict__ halo,
-                                         int hcap, int tile, int w, int g, int j) {
+    __device__ __forceinline__ void load(const float* __restrict__ u, int K, const int* __restrict__ halo, int hcap,
+                                         int tile, int w, int g, int j) {
         int hid[HPW];
         load_ids(halo, hcap, tile, w, j, hid);
-        load_data(u, K, Kp, hcap, tile, w, g, j, hid);
+        load_data(u, K, hcap, tile, w, g, j, hid);
     }
 };
-
-// The RK update runs in the product output layout (lane j + 16g holds
-// modes 4g..4g+3 of element j) while states are read in the operand layout
-// (lane j + 16k holds mode 4s+k of block s).  For a stage whose input is
-// the RK base state itself (RK4 stage 1, RK2 stage 1) the base state is put
-// in the output layout from the registers already loaded, with products
-// against 0/1 selector operands (exact: one product of 1*x per entry), so
-// it is not read from memory a second time.
-template <int N>
-__device__ __forceinline__ void operand_to_rows(const float (&ub)[N][4], int lane, float (&rows)[N][4]) {
-    const int i = lane & 15, k = lane >> 4;
-#pragma unroll
-    for (int v = 0; v < N; v++) {
-        floatx4 d = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 4; s++) d = mfma4(i == 4 * s + k ? 1.f : 0.f, ub[v][s], d);
-#pragma unroll
-        for (int r = 0; r < 4; r++) rows[v][r] = d[r];
-    }
-}
 
 // RK4 on the matrix-core path keeps acc = u_n + k1/6 + k2/3 + k3/3 (the
 // base state folded in), so the last stage writes acc + k4/6 without
@@ -341,31 +321,30 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
 
     // inverse mass and the RK stage update; lane holds modes m = 4g+i
     using RK = RkNeeds<KIND>;
-    const unsigned lane_rk = (unsigned)(4 * g * Kp + ee);
-    const unsigned lane_st = (unsigned)(4 * g * Kp + e);
+    constexpr int VS = tile_var_stride<P>();
+    const unsigned l4 = tile_lane4<N, P>(ee, g), s4 = tile_lane4<N, P>(e, g);
     const float cj = __builtin_amdgcn_rcpf(Jv) * dt;
-    float cin[N][4];
-    if constexpr (RK::C_FROM_INPUT) operand_to_rows<N>(st.ub, lane, cin);
     float c0[N];
     double lam = 0.0;
 #pragma unroll
-    for (int v = 0; v < N; v++)
+    for (int v = 0; v < N; v++) {
+        floatx4 c = {0.f, 0.f, 0.f, 0.f}, aa = {0.f, 0.f, 0.f, 0.f}, o, an;
+        if constexpr (RK::C_FROM_INPUT) c = st.ub[v];
+        if constexpr (RK::LOAD_C) c = gp4(a.c)[v * VS + l4];
+        if constexpr (RK::LOAD_ACC) aa = gp4(a.acc)[v * VS + l4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const bool mv = 4 * g + i < NP;
-            const size_t uoff = (size_t)(v * NP + i) * Kp;
-            float c = 0.f;
-            if constexpr (RK::C_FROM_INPUT) c = cin[v][i];
-            if constexpr (RK::LOAD_C) c = mv ? (gp(a.c) + uoff)[lane_rk] : 0.f;
-            const float aa = (RK::LOAD_ACC && mv) ? (gp(a.acc) + uoff)[lane_rk] : 0.f;
-            if (mv && valid) {
-                float an = 0.f;
-                const float o = rk_update<KIND>(c, aa, cj * R[v][i], &an);
-                if constexpr (rk_writes_acc<KIND>()) (gp(a.acc) + uoff)[lane_st] = an;
-                (gp(a.out) + uoff)[lane_st] = o;
-                if (i == 0) c0[v] = o;
-            }
+            float t = 0.f;
+            o[i] = rk_update<KIND>(c[i], aa[i], cj * R[v][i], &t);
+            an[i] = t;
         }
+        // padding elements (e >= K) compute a copy of element K-1 into their own slots
+        if (tile_groups(NP) == 4 || g < tile_groups(NP)) {
+            if constexpr (rk_writes_acc<KIND>()) gp4(a.acc)[v * VS + s4] = an;
+            gp4(a.out)[v * VS + s4] = o;
+        }
+        c0[v] = o[0];
+    }
     if constexpr (LAST) {
         if (g == 0 && valid) {
             double Uc[4];
@@ -428,7 +407,7 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
     using RK = RkNeeds<KIND>;
     constexpr int HPW = MfmaTileState<N, P>::HPW;
     static_assert(NP <= 16 && NQ <= 16 && NPT <= 16, "order too high for 16x16 tiles");
-    const int K = a.K, Kp = a.Kp;
+    const int K = a.K;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = lane >> 4, j = lane & 15;
     const int npairs = (mx.hcap >> 4) * N;
@@ -483,23 +462,22 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
         U[v] = acc;
     }
     // RK registers of this tile, then the next tile's state (st is dead
-    // after this block).  The loads are unconditional (row clamped into the
-    // array for the padding modes m >= n_p, whose values are never used), so
-    // no branches sit between them and the waits that follow.
-    float cr[N][4], ar[N][4];
-    if constexpr (RK::C_FROM_INPUT) operand_to_rows<N>(st.ub, lane, cr);
+    // after this block).  For a stage whose input is the RK base state the
+    // operand registers already hold it in the output layout.
+    constexpr int VS = tile_var_stride<P>();
+    const unsigned l4 = tile_lane4<N, P>(ee, g);
+    floatx4 cr[N], ar[N];
 #pragma unroll
-    for (int v = 0; v < N; v++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const unsigned row = (unsigned)min(v * NP + 4 * g + i, N * NP - 1);
-            const unsigned ix = row * (unsigned)Kp + (unsigned)ee;
-            if constexpr (!RK::C_FROM_INPUT) cr[v][i] = RK::LOAD_C ? gp(a.c)[ix] : 0.f;
-            ar[v][i] = RK::LOAD_ACC ? gp(a.acc)[ix] : 0.f;
-        }
+    for (int v = 0; v < N; v++) {
+        cr[v] = floatx4{0.f, 0.f, 0.f, 0.f};
+        ar[v] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if constexpr (RK::C_FROM_INPUT) cr[v] = st.ub[v];
+        if constexpr (RK::LOAD_C) cr[v] = gp4(a.c)[v * VS + l4];
+        if constexpr (RK::LOAD_ACC) ar[v] = gp4(a.acc)[v * VS + l4];
+    }
     __builtin_amdgcn_sched_barrier(0);
     if (next >= 0) {
-        st.load_data(a.u, K, Kp, mx.hcap, next, w, g, j, hid);
+        st.load_data(a.u, K, mx.hcap, next, w, g, j, hid);
     }
     __builtin_amdgcn_sched_barrier(0);
     floatx4 R[N];
@@ -561,26 +539,26 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
     }
 
     // inverse mass and the RK stage update; lane holds modes m = 4g+i
-    const unsigned lane_st = (unsigned)(4 * g * Kp + e);
+    const unsigned s4 = tile_lane4<N, P>(e, g);
     const float cj = __builtin_amdgcn_rcpf(Jv) * dt;
     float c0[N];
     double lam = 0.0;
 #pragma unroll
-    for (int v = 0; v < N; v++)
+    for (int v = 0; v < N; v++) {
+        floatx4 o, an;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const bool mv = 4 * g + i < NP;
-            const size_t uoff = (size_t)(v * NP + i) * Kp;
-            const float c = cr[v][i];
-            const float aa = ar[v][i];
-            if (mv && valid) {
-                float an = 0.f;
-                const float o = rk_update<KIND>(c, aa, cj * R[v][i], &an);
-                if constexpr (rk_writes_acc<KIND>()) (gp(a.acc) + uoff)[lane_st] = an;
-                (gp(a.out) + uoff)[lane_st] = o;
-                if (i == 0) c0[v] = o;
-            }
+            float t = 0.f;
+            o[i] = rk_update<KIND>(cr[v][i], ar[v][i], cj * R[v][i], &t);
+            an[i] = t;
         }
+        // padding elements (e >= K) compute a copy of element K-1 into their own slots
+        if (tile_groups(NP) == 4 || g < tile_groups(NP)) {
+            if constexpr (rk_writes_acc<KIND>()) gp4(a.acc)[v * VS + s4] = an;
+            gp4(a.out)[v * VS + s4] = o;
+        }
+        c0[v] = o[0];
+    }
     if constexpr (LAST) {
         if (g == 0 && valid) {
             double Uc[4];

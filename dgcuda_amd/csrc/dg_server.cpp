@@ -33,7 +33,8 @@ namespace dgmi {
 static int64_t estimate_bytes(int K, int N, int p, int precision) {
     const int64_t Kp = (K + 63) / 64 * 64;
     const int64_t s = precision == DG_FP64 ? 8 : 4;
-    const int64_t np = n_modes(p);
+    // FP32 at p <= 4 reserves the matrix-core tile layout (modes padded to a multiple of 4)
+    const int64_t np = (precision == DG_FP32 && p >= 1 && p <= 4) ? 4 * tile_groups(n_modes(p)) : n_modes(p);
     const int64_t ntiles = (K + 63) / 64;
     return Kp * (4 * N * np * s + 14 * s + 3 * 4 + 6 * 8 + 8 + 8 + 3 * 2) + ntiles * 64 * 4 + 4096;
 }

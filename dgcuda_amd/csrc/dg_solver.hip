@@ -223,6 +223,10 @@ class SolverT : public Solver {
     int graph_scheme = 0;
     int64_t bytes = 0;
     int audit_blocks = 0;
+    // layout of c, acc, ua, ub: mode-major for the lane kernel, the tile
+    // layout for the matrix-core kernels (FP32); switched with the kernel
+    StateIx six{};
+    bool state_set = false;  // c holds a state (projected, set or stepped)
     double b0 = 0;  // phi_0 at the first volume point (cell mean = c_0 * b0)
 
     ~SolverT() override {
@@ -233,7 +237,11 @@ class SolverT : public Solver {
         pool_release_stream(stream);
     }
 
-    size_t state_elems() const { return (size_t)N * NP * Kp; }
+    // entries allocated per state array: room for either layout
+    size_t state_alloc() const {
+        StateIx t{N, NP, Kp, sizeof(R) == 4 && mfma_supported(N, p) ? 1 : 0};
+        return t.elems();
+    }
 
     Status alloc(void** p, size_t n) {
         Status st = pool_alloc(p, n);
@@ -271,7 +279,8 @@ class SolverT : public Solver {
         Status st = ensure_device_ready();
         if (!st) return st;
         if (!(st = pool_stream(&stream))) return st;
-        const size_t M = state_elems();
+        six = StateIx{N, NP, Kp, 0};
+        const size_t M = state_alloc();
         if (!(st = alloc((void**)&c, M * sizeof(R)))) return st;
         if (!(st = alloc((void**)&acc, M * sizeof(R)))) return st;
         if (!(st = alloc((void**)&ua, M * sizeof(R)))) return st;
@@ -418,7 +427,8 @@ class SolverT : public Solver {
                 return Status::ok();
             }
         }
-        HIPST(hipMemsetAsync(c, 0, state_elems() * sizeof(R), stream));
+        // timed on the scratch buffers (zeros), so a held state is untouched
+        HIPST(hipMemsetAsync(ub, 0, state_alloc() * sizeof(R), stream));
         HIPST(launch_set_dt(ctrl, 0.0, 0, stream));  // activates the stage kernels
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIPST(hipEventCreate(&e0));
@@ -431,9 +441,9 @@ class SolverT : public Solver {
             LaunchShape cand;
             if (!make_shape(kind, cand)) continue;
             shape = cand;
-            if (!(st = stage(ST_RHS, c, c, ub))) break;
+            if (!(st = stage(ST_RHS, ub, ub, ua))) break;
             if (hipEventRecord(e0, stream) != hipSuccess) break;
-            for (int r = 0; r < 3 && st; r++) st = stage(ST_RHS, c, c, ub);
+            for (int r = 0; r < 3 && st; r++) st = stage(ST_RHS, ub, ub, ua);
             if (!st) break;
             float ms = 0.f;
             if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -472,6 +482,20 @@ class SolverT : public Solver {
             if (req != 0) return st;
             // automatic choice falls back to the lane kernel
             if (!(st = make_shape(1, s))) return st;
+        }
+        const StateIx nix{N, NP, Kp, (sizeof(R) == 4 && s.kind >= 2) ? 1 : 0};
+        if (nix.tiled != six.tiled) {
+            // acc, ua and ub are written in full (padding included) by a
+            // stage before any stage reads them; c is carried over
+            if (state_set) {
+                std::vector<double> h((size_t)N * NP * K);
+                if (!(st = download_ref(c, h.data()))) return st;
+                six = nix;
+                if (!(st = upload_ref(h.data(), c))) return st;
+            } else {
+                six = nix;
+                HIPST(hipMemsetAsync(c, 0, state_alloc() * sizeof(R), stream));
+            }
         }
         shape = s;
         lam_valid = false;
@@ -535,7 +559,7 @@ class SolverT : public Solver {
 
     Status ensure_lambda() {
         if (lam_valid) return Status::ok();
-        HIPST(launch_lambda<R>(N, c, K, Kp, NP, b0, phys, ctrl, stream));
+        HIPST(launch_lambda<R>(c, six, K, b0, phys, ctrl, stream));
         lam_valid = true;
         return Status::ok();
     }
@@ -598,22 +622,24 @@ class SolverT : public Solver {
     }
 
     Status project_ic() override {
-        HIPST(launch_project_ic<R>(N, p, c, V, K, Kp, ic, stream));
+        HIPST(launch_project_ic<R>(p, c, six, V, K, ic, stream));
         HIPST(hipStreamSynchronize(stream));
+        state_set = true;
         t = 0;
         steps = 0;
         lam_valid = false;
         return Status::ok();
     }
 
+    // c_ref[var][mode][elem] (FP64, caller's element order) -> device, in
+    // the current layout; padding elements replicate the last element and
+    // padding modes are zero
     Status upload_ref(const double* c_ref, R* dst) {
-        std::vector<R> h(state_elems(), R(0));
+        std::vector<R> h(six.elems(), R(0));
         for (int v = 0; v < N; v++)
             for (int m = 0; m < NP; m++) {
                 const double* src = c_ref + ((size_t)v * NP + m) * K;
-                R* d = h.data() + ((size_t)v * NP + m) * Kp;
-                for (int i = 0; i < K; i++) d[i] = (R)src[perm[i]];
-                for (int i = K; i < Kp; i++) d[i] = d[K - 1];
+                for (int i = 0; i < Kp; i++) h[six.at(v, m, i)] = (R)src[perm[std::min(i, K - 1)]];
             }
         HIPST(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice, stream));
         HIPST(hipStreamSynchronize(stream));
@@ -621,20 +647,20 @@ class SolverT : public Solver {
     }
 
     Status download_ref(const R* src, double* c_ref) {
-        std::vector<R> h(state_elems());
+        std::vector<R> h(six.elems());
         HIPST(hipMemcpyAsync(h.data(), src, h.size() * sizeof(R), hipMemcpyDeviceToHost, stream));
         HIPST(hipStreamSynchronize(stream));
         for (int v = 0; v < N; v++)
             for (int m = 0; m < NP; m++) {
                 double* d = c_ref + ((size_t)v * NP + m) * K;
-                const R* s = h.data() + ((size_t)v * NP + m) * Kp;
-                for (int i = 0; i < K; i++) d[perm[i]] = (double)s[i];
+                for (int i = 0; i < K; i++) d[perm[i]] = (double)h[six.at(v, m, i)];
             }
         return Status::ok();
     }
 
     Status set_state(const double* c_ref) override {
         Status st = upload_ref(c_ref, c);
+        state_set = (bool)st;
         t = 0;
         steps = 0;
         lam_valid = false;
@@ -710,7 +736,7 @@ class SolverT : public Solver {
     }
 
     Status audit(dg_audit_out* out) override {
-        HIPST(launch_audit<R>(N, c, Jd, K, Kp, NP, b0, phys.gamma, audit_part, audit_blocks, stream));
+        HIPST(launch_audit<R>(c, six, Jd, K, b0, phys.gamma, audit_part, audit_blocks, stream));
         std::vector<double> h((size_t)audit_blocks * 8);
         HIPST(hipMemcpyAsync(h.data(), audit_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPST(hipStreamSynchronize(stream));
@@ -733,7 +759,7 @@ class SolverT : public Solver {
 
     Status eval_vertices(int var, double* uv) override {
         if (var < 0 || var >= N) return Status::err(DG_ERR_ARG, "variable index out of range");
-        std::vector<double> cr(state_elems());
+        std::vector<double> cr((size_t)N * NP * K);
         Status st = get_state(cr.data());
         if (!st) return st;
         Tables tb = build_tables(p);

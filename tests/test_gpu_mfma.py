@@ -78,3 +78,36 @@ def test_autotuned_solver_runs_correctly():
         err = orc.rel_l2(s.get_state(), ref)
         print("p=%d kernel %d rel L2 %.3e" % (p, s.kernel(), err))
         assert err <= 1e-5
+
+
+@pytest.mark.parametrize("p", [1, 3, 4])
+def test_kernel_switch_carries_state(p):
+    """The matrix-core kernels keep the FP32 state in their own tile layout
+    (modes padded to a multiple of 4; p = 1, 3 pad, p = 4 pads one mode).
+    Switching the kernel of a solver that holds a state converts it exactly,
+    and steps taken after a switch equal steps on that kernel from the start
+    (bitwise).  K = 162: the last tile is partial."""
+    print("kernel switch p=%d" % p, flush=True)
+    m = shuffled_mesh(9, seed=5)
+    prob = problem_for(4)
+    o = orc.Oracle.from_mesh(m, p, prob)
+    u = perturbed_state(o, prob, m.period, seed=11)
+    u32 = u.astype(np.float32).astype(np.float64)
+    s = dg.Solver(m, prob, p, dg.DG_FP32)
+    s.set_kernel(1)
+    s.set_state(u)
+    for kind in (2, 1, 3, 2, 1):
+        s.set_kernel(kind)
+        assert s.kernel() == kind
+        assert np.array_equal(s.get_state(), u32), "state changed by the switch to kernel %d" % kind
+    s.set_kernel(2)
+    s.run_steps(3, 0.3)
+    b = dg.Solver(m, prob, p, dg.DG_FP32)
+    b.set_kernel(2)
+    b.set_state(u)
+    b.run_steps(3, 0.3)
+    got = s.get_state()
+    assert np.isfinite(got).all()
+    assert np.array_equal(got, b.get_state())
+    s.set_kernel(1)
+    assert np.array_equal(s.get_state(), got)
