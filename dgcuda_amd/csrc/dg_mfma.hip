@@ -70,18 +70,29 @@ __global__ void __launch_bounds__(256, DGMI_MFMA_MINW) k_stage_mfma(StageArgs<fl
     const int g = lane >> 4, j = lane & 15;
     double lam = 0.0;
     __shared__ float4 tabA[MFMA_A_SLOTS * 64];
+    const float dt = (float)a.ctrl->dt;
     MfmaTileState<N, P> st;
     int hid[MfmaTileState<N, P>::HPW];
+    TileGeo gc;
+    // tiles past the end of the walk read tile tend-1 instead (the loads
+    // stay unconditional and in bounds; nothing is computed from them)
+    auto walk = [&](int x) { return min(x, tend - 1); };
     if (t < tend) {
         MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, t, w, j, hid);
-        st.load_data(a.u, a.K, mx.hcap, t, w, g, j, hid);
+        gc.load(a, mx, t, w * 16 + j);
+        st.load_data(a.u, a.K, t, w, g, j, hid);
+        MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, walk(t + stride), w, j, hid);
     }
     if (w == 0) mfma_a_to_lds<N, P>(tabA, lane);
+    // The first tile needs these loads at once.  Waiting for them here also
+    // keeps the compiler's count of loads in flight at the loop head to the
+    // steady state (loop back edge), where the next tile's operands were
+    // issued before the previous tile's stores and need not wait for them.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (; t < tend; t += stride) {
-        const int tn = t + stride;
-        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND>(a, mx, tabA, st, hid, t, tn < tend ? tn : -1, fd, ts));
-    }
+    for (int tc = t; tc < tend; tc += stride)
+        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND>(a, mx, tabA, st, hid, gc, dt, tc, walk(tc + stride),
+                                                           walk(tc + 2 * stride), fd, ts));
 
     if constexpr (LAST) {
         const double m2 = block_max(lam, shmax);

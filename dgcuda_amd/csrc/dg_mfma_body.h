@@ -120,35 +120,32 @@ struct MfmaTileState {
     // Halo element ids of this wave's pairs.  A halo state load depends on
     // its id, and waiting for an id waits for every load issued before it,
     // so the ids are requested ahead of the state loads that use them.
+    // Ids and halo states are loaded for every slot q, pairs past the list
+    // from its last entries (in bounds, unused): loads under a branch, with
+    // registers zeroed on the other path, make the compiler wait for them
+    // (and for everything issued before them) at the next write.
     __device__ __forceinline__ static void load_ids(const int* __restrict__ halo, int hcap, int tile, int w, int j,
                                                    int (&hid)[HPW]) {
-        const int npairs = (hcap >> 4) * N;
+        const int hlast = (hcap >> 4) - 1;
 #pragma unroll
         for (int q = 0; q < HPW; q++) {
-            const int pr = w + 4 * q;
-            hid[q] = (pr < npairs) ? gp(halo)[(unsigned)(tile * hcap + (pr / N) * 16 + j)] : 0;
+            const int hs = min((w + 4 * q) / N, hlast);
+            hid[q] = gp(halo)[(unsigned)(tile * hcap + hs * 16 + j)];
         }
     }
-    __device__ __forceinline__ void load_data(const float* __restrict__ u, int K, int hcap, int tile, int w, int g,
-                                              int j, const int (&hid)[HPW]) {
+    __device__ __forceinline__ void load_data(const float* __restrict__ u, int K, int tile, int w, int g, int j,
+                                              const int (&hid)[HPW]) {
         const unsigned l4 = tile_lane4<N, P>(min(tile * MFMA_TILE + w * 16 + j, K - 1), g);
 #pragma unroll
         for (int v = 0; v < N; v++) ub[v] = gp4(u)[v * VS + l4];
-        const int npairs = (hcap >> 4) * N;
 #pragma unroll
-        for (int q = 0; q < HPW; q++) {
-            const int pr = w + 4 * q;
-            if (pr < npairs)
-                hb[q] = gp4(u)[(pr % N) * VS + tile_lane4<N, P>(hid[q], g)];
-            else
-                hb[q] = floatx4{0.f, 0.f, 0.f, 0.f};
-        }
+        for (int q = 0; q < HPW; q++) hb[q] = gp4(u)[((w + 4 * q) % N) * VS + tile_lane4<N, P>(hid[q], g)];
     }
     __device__ __forceinline__ void load(const float* __restrict__ u, int K, const int* __restrict__ halo, int hcap,
                                          int tile, int w, int g, int j) {
         int hid[HPW];
         load_ids(halo, hcap, tile, w, j, hid);
-        load_data(u, K, hcap, tile, w, g, j, hid);
+        load_data(u, K, tile, w, g, j, hid);
     }
 };
 
@@ -360,14 +357,15 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
 
 // Geometry of one tile: the (face normal, length, packed neighbour word)
 // entry this thread stores to LDS (threads 0..191 hold one of the 64 x 3
-// entries) and the affine map of the lane's own element.
+// entries; the others load entry 191 again and store nothing, so the loads
+// are not under a branch) and the affine map of the lane's own element.
 struct TileGeo {
     float4 f;
     float xr, yr, xs, ys, J;
     __device__ __forceinline__ void load(const StageArgs<float>& a, const MfmaArgs& mx, int tile, int el) {
         const int K = a.K, Kp = a.Kp;
-        const int i = threadIdx.x;
-        if (i < MFMA_TILE * 3) {
+        {
+            const int i = min((int)threadIdx.x, MFMA_TILE * 3 - 1);
             const int l = i / 3, fc = i - 3 * l;
             const unsigned e2 = (unsigned)min(tile * MFMA_TILE + l, K - 1);
             const DG_GLOBAL float* gf = gp(a.geo_f) + (size_t)(3 * fc) * Kp;
@@ -386,20 +384,24 @@ struct TileGeo {
     }
 };
 
-// Software-pipelined body of the persistent single-solve kernel: `st`
-// holds this tile's state on entry and `hid` the halo ids of tile `next`
-// are requested at the start.  Once the volume interpolation has consumed
-// the state, the tile's RK registers (c, acc) are requested first and then
-// the state of tile `next` (if >= 0) into the same registers, so the RK
-// update waits only for its own loads and the next tile's loads overlap the
-// flux work, the update and the stores.  The vector-memory counter retires
-// in issue order, hence the ordering (and the scheduling barriers that
-// keep it).  The A operands are read from the LDS table `tabA`.
+// Software-pipelined body of the persistent single-solve kernel.  On entry
+// `st` and `gc` hold the state and geometry of `tile`, and `hid` the halo
+// ids of `next`.  Once the volume interpolation has consumed the state, the
+// tile's RK registers (c, acc) are requested and then the state of `next`
+// into the same registers; after the face fluxes the geometry of `next` and
+// the halo ids of `next2` are requested, ahead of this tile's stores.  The
+// vector-memory counter retires in issue order and counts stores, so every
+// load a later wait is for is issued before the stores that wait need not
+// cover (the scheduling barriers keep that order).  `next` and `next2` are
+// tiles of this launch (the caller repeats the last tile where the walk
+// ends), so no load sits under a branch.  The A operands are read from the
+// LDS table `tabA`.
 template <int N, int P, int KIND>
 __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>& a, const MfmaArgs& mx,
                                                          const float4* tabA, MfmaTileState<N, P>& st,
-                                                         int (&hid)[MfmaTileState<N, P>::HPW], int tile, int next,
-                                                         float4* fd, float* ts) {
+                                                         int (&hid)[MfmaTileState<N, P>::HPW], TileGeo& gc,
+                                                         float dt, int tile, int next, int next2, float4* fd,
+                                                         float* ts) {
     constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1, NPT = 3 * NQ1;
     constexpr int TILE = MFMA_TILE;
     constexpr int TS = 17 * N;
@@ -411,18 +413,13 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = lane >> 4, j = lane & 15;
     const int npairs = (mx.hcap >> 4) * N;
-    const float dt = (float)a.ctrl->dt;
     const int base = tile * TILE;
     const int el = w * 16 + j;
     const int e = base + el;
     const bool valid = e < K;
     const int ee = valid ? e : K - 1;
 
-    // the tile's geometry, requested first: it is needed only after phase 1
-    TileGeo gc;
-    gc.load(a, mx, tile, el);
-    if (next >= 0) MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, next, w, j, hid);
-    const float xr = gc.xr, yr = gc.yr, xs = gc.xs, ys = gc.ys, Jv = gc.J;
+    const float xr = gc.xr, yr = gc.yr, xs = gc.xs, ys = gc.ys;
 
     // ---- phase 1: face traces of the own sub-tile and of this wave's halo pairs ----
     const float4 Atr = tabA[MFMA_A_TR * 64 + lane];
@@ -476,9 +473,7 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
         if constexpr (RK::LOAD_ACC) ar[v] = gp4(a.acc)[v * VS + l4];
     }
     __builtin_amdgcn_sched_barrier(0);
-    if (next >= 0) {
-        st.load_data(a.u, K, mx.hcap, next, w, g, j, hid);
-    }
+    st.load_data(a.u, K, next, w, g, j, hid);
     __builtin_amdgcn_sched_barrier(0);
     floatx4 R[N];
 #pragma unroll
@@ -538,9 +533,16 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
         for (int v = 0; v < N; v++) R[v] = mfma4(Asp[i], G[v], R[v]);
     }
 
+    // geometry of the next tile and halo ids of the one after, requested
+    // before this tile's stores
+    const float cj = __builtin_amdgcn_rcpf(gc.J) * dt;
+    __builtin_amdgcn_sched_barrier(0);
+    gc.load(a, mx, next, el);
+    MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, next2, w, j, hid);
+    __builtin_amdgcn_sched_barrier(0);
+
     // inverse mass and the RK stage update; lane holds modes m = 4g+i
     const unsigned s4 = tile_lane4<N, P>(e, g);
-    const float cj = __builtin_amdgcn_rcpf(Jv) * dt;
     float c0[N];
     double lam = 0.0;
 #pragma unroll
