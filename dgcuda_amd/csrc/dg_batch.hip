@@ -201,6 +201,8 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float
         MfmaArgs mx;
         mx.fword = uni(sd->fword);
         mx.halo = uni(sd->halo);
+        mx.face4 = nullptr;  // the batch body reads geo_f and fword
+        mx.metric4 = nullptr;
         mx.hcap = uni(sd->hcap);
         mx.pad = 0;
         const int N = uni(sd->N), p = uni(sd->p);
@@ -288,6 +290,7 @@ class EngineT : public BatchEngine {
     hipGraphExec_t gexec = nullptr;
     int solo_min = 0;             // 0: every solve in the batch launch
     std::vector<Solver*> solo;    // solves launched with their own kernels
+    std::vector<uint64_t> solo_uid;  // their ids: the captured step holds their buffers
 
     ~EngineT() override {
         if (st) (void)hipStreamSynchronize(st);
@@ -421,9 +424,15 @@ class EngineT : public BatchEngine {
         HIPST(hipStreamSynchronize(st));  // the host vectors go out of scope
         nslots = ns;
         nitems = ni;
-        if (new_solo != solo) {
+        // The solo launches are part of the captured step, with the buffers of
+        // the solves they were captured for; compare by id, since a new solve
+        // can sit at the address of a deleted one.
+        std::vector<uint64_t> new_uid;
+        for (Solver* sv : new_solo) new_uid.push_back(sv->uid);
+        if (new_uid != solo_uid) {
             solo = new_solo;
-            drop_graph();  // the solo launches are part of the captured step
+            solo_uid = new_uid;
+            drop_graph();
         }
         return Status::ok();
     }

@@ -28,13 +28,20 @@ hipError_t launch_stage_valu(int N, int p, const StageArgs<R>& a, dim3 grid, hip
 // Extra per-solve data of the matrix-core kernel: per (element, face) a
 // 16-bit word {bits 0..11 LDS slot of the neighbour (own tile element or
 // halo entry), bits 12..13 neighbour's local face, bit 14 left element},
-// stored [3][Kp]; per tile the halo element list [ntiles][hcap].
+// stored [3][Kp]; per tile the halo element list [ntiles][hcap]; and the
+// packed geometry of the single-solve kernel: per (element, face) one
+// float4 (nx, ny, len, word as bits) [Kp][3], per element one float4
+// (xr, yr, xs, ys) [Kp], padding elements replicating element K-1.
 struct MfmaArgs {
     const uint16_t* fword;
     const int* halo;
+    const float4* face4;
+    const float4* metric4;
     int hcap;
     int pad;
 };
+hipError_t launch_mfma_pack(const float* geo_e, const float* geo_f, const uint16_t* fword, int K, int Kp,
+                            float4* face4, float4* metric4, hipStream_t st);
 
 // matrix-core element kernel (FP32, p = 1..4, state in the tile layout):
 // 64-element tiles, 4 wavefronts per workgroup, persistent over the tiles

@@ -102,6 +102,28 @@ __global__ void __launch_bounds__(256, DGMI_MFMA_MINW) k_stage_mfma(StageArgs<fl
 
 size_t mfma_lds_bytes(int N, int hcap) { return mfma_tile_lds_bytes(N, hcap); }
 
+// Packs the geometry of the matrix-core kernel (one thread per element):
+// a tile's 192 face records and 64 element maps become contiguous 16-byte
+// entries.
+__global__ void __launch_bounds__(256) k_mfma_pack(const float* __restrict__ geo_e, const float* __restrict__ geo_f,
+                                                  const uint16_t* __restrict__ fword, int K, int Kp,
+                                                  float4* __restrict__ face4, float4* __restrict__ metric4) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Kp) return;
+    const size_t s = (size_t)min(e, K - 1), kp = (size_t)Kp;
+    for (int f = 0; f < 3; f++)
+        face4[3 * (size_t)e + f] = make_float4(geo_f[3 * f * kp + s], geo_f[(3 * f + 1) * kp + s],
+                                               geo_f[(3 * f + 2) * kp + s], __int_as_float((int)fword[f * kp + s]));
+    metric4[e] = make_float4(geo_e[s], geo_e[kp + s], geo_e[2 * kp + s], geo_e[3 * kp + s]);
+}
+
+hipError_t launch_mfma_pack(const float* geo_e, const float* geo_f, const uint16_t* fword, int K, int Kp,
+                            float4* face4, float4* metric4, hipStream_t st) {
+    hipLaunchKernelGGL(k_mfma_pack, dim3((Kp + 255) / 256), dim3(256), 0, st, geo_e, geo_f, fword, K, Kp, face4,
+                       metric4);
+    return hipGetLastError();
+}
+
 // workgroups of the matrix-core kernel that fit on the device at once
 int mfma_resident_groups(int N, int p, int hcap) {
     const void* fn = nullptr;

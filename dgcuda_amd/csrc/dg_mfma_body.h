@@ -355,32 +355,26 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
     return lam;
 }
 
-// Geometry of one tile: the (face normal, length, packed neighbour word)
-// entry this thread stores to LDS (threads 0..191 hold one of the 64 x 3
-// entries; the others load entry 191 again and store nothing, so the loads
-// are not under a branch) and the affine map of the lane's own element.
+// Geometry of one tile from the packed arrays: the (face normal, length,
+// packed neighbour word) record this thread stores to LDS (threads 0..191
+// hold one of the 64 x 3 records; the others load record 191 again and
+// store nothing, so the load is not under a branch), the affine map and
+// the Jacobian of the lane's own element.  Padding elements replicate
+// element K-1 in every geometry array.
 struct TileGeo {
     float4 f;
     float xr, yr, xs, ys, J;
     __device__ __forceinline__ void load(const StageArgs<float>& a, const MfmaArgs& mx, int tile, int el) {
-        const int K = a.K, Kp = a.Kp;
-        {
-            const int i = min((int)threadIdx.x, MFMA_TILE * 3 - 1);
-            const int l = i / 3, fc = i - 3 * l;
-            const unsigned e2 = (unsigned)min(tile * MFMA_TILE + l, K - 1);
-            const DG_GLOBAL float* gf = gp(a.geo_f) + (size_t)(3 * fc) * Kp;
-            f.x = gf[e2];
-            f.y = (gf + Kp)[e2];
-            f.z = (gf + 2 * (size_t)Kp)[e2];
-            f.w = __int_as_float((int)(gp(mx.fword) + (size_t)fc * Kp)[e2]);
-        }
-        const unsigned ueu = (unsigned)min(tile * MFMA_TILE + el, K - 1);
-        const DG_GLOBAL float* ge = gp(a.geo_e);
-        xr = ge[ueu];
-        yr = (ge + Kp)[ueu];
-        xs = (ge + 2 * (size_t)Kp)[ueu];
-        ys = (ge + 3 * (size_t)Kp)[ueu];
-        J = (ge + 4 * (size_t)Kp)[ueu];
+        const floatx4 r = ((const DG_GLOBAL floatx4*)mx.face4)[(unsigned)(tile * (MFMA_TILE * 3) +
+                                                                            min((int)threadIdx.x, MFMA_TILE * 3 - 1))];
+        f = make_float4(r[0], r[1], r[2], r[3]);
+        const unsigned ue = (unsigned)(tile * MFMA_TILE + el);
+        const floatx4 m = ((const DG_GLOBAL floatx4*)mx.metric4)[ue];
+        xr = m[0];
+        yr = m[1];
+        xs = m[2];
+        ys = m[3];
+        J = (gp(a.geo_e) + 4 * (size_t)a.Kp)[ue];
     }
 };
 
@@ -422,15 +416,25 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
     const float xr = gc.xr, yr = gc.yr, xs = gc.xs, ys = gc.ys;
 
     // ---- phase 1: face traces of the own sub-tile and of this wave's halo pairs ----
+    // (points 4g..4g+3 of every lane are written: a slot holds 16 points plus
+    // a padding row, and points >= 3(p+1) are never read)
     const float4 Atr = tabA[MFMA_A_TR * 64 + lane];
+    {
+        floatx4 T[N];
 #pragma unroll
-    for (int v = 0; v < N; v++) {
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int v = 0; v < N; v++) {
+            T[v] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < 4; s++) acc = mfma4(Atr[s], st.ub[v][s], acc);
+            for (int s = 0; s < 4; s++) T[v] = mfma4(Atr[s], st.ub[v][s], T[v]);
+        }
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (4 * g + i < NPT) ts[el * TS + (4 * g + i) * N + v] = acc[i];
+        for (int i = 0; i < 4; i++) {
+            float* dst = ts + el * TS + (4 * g + i) * N;
+            if constexpr (N == 4)
+                *reinterpret_cast<floatx4*>(dst) = floatx4{T[0][i], T[1][i], T[2][i], T[3][i]};
+            else
+                dst[0] = T[0][i];
+        }
     }
 #pragma unroll
     for (int q = 0; q < HPW; q++) {
@@ -441,8 +445,7 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
 #pragma unroll
             for (int s = 0; s < 4; s++) acc = mfma4(Atr[s], st.hb[q][s], acc);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (4 * g + i < NPT) ts[slot * TS + (4 * g + i) * N + v] = acc[i];
+            for (int i = 0; i < 4; i++) ts[slot * TS + (4 * g + i) * N + v] = acc[i];
         }
     }
     if (threadIdx.x < TILE * 3) fd[threadIdx.x] = gc.f;

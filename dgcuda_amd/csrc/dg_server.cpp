@@ -36,7 +36,9 @@ static int64_t estimate_bytes(int K, int N, int p, int precision) {
     // FP32 at p <= 4 reserves the matrix-core tile layout (modes padded to a multiple of 4)
     const int64_t np = (precision == DG_FP32 && p >= 1 && p <= 4) ? 4 * tile_groups(n_modes(p)) : n_modes(p);
     const int64_t ntiles = (K + 63) / 64;
-    return Kp * (4 * N * np * s + 14 * s + 3 * 4 + 6 * 8 + 8 + 8 + 3 * 2) + ntiles * 64 * 4 + 4096;
+    // matrix-core tiles (FP32, p >= 3 in a batch): packed geometry, 64 bytes per element
+    const int64_t packed = (precision == DG_FP32 && p >= 3) ? 64 : 0;
+    return Kp * (4 * N * np * s + 14 * s + 3 * 4 + 6 * 8 + 8 + 8 + 3 * 2 + packed) + ntiles * 64 * 4 + 4096;
 }
 
 class ServerImpl : public Server {

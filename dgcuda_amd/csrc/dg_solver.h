@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -68,6 +69,12 @@ struct SlotInfo {
 
 class Solver {
    public:
+    // process-unique, never reused (a new solve may reuse a freed one's address)
+    const uint64_t uid = next_uid();
+    static uint64_t next_uid() {
+        static std::atomic<uint64_t> n{0};
+        return ++n;
+    }
     virtual ~Solver() {}
     virtual void slot_info(SlotInfo* out) = 0;
     // pushes the run parameters to the control block and computes the wave

@@ -212,6 +212,8 @@ class SolverT : public Solver {
     std::vector<int> h_nbr[3];
     uint16_t* fword = nullptr;
     int* halo = nullptr;
+    float4 *face4 = nullptr, *metric4 = nullptr;
+    int64_t tile_bytes = 0;  // fword, halo and packed geometry
     MfmaArgs mx{};
     double *V = nullptr, *Jd = nullptr, *audit_part = nullptr;
     Ctrl* ctrl = nullptr;
@@ -232,7 +234,7 @@ class SolverT : public Solver {
     ~SolverT() override {
         if (stream) (void)hipStreamSynchronize(stream);  // nothing queued may use the buffers
         if (gexec) (void)hipGraphExecDestroy(gexec);
-        void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, audit_part, ctrl, fword, halo};
+        void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, audit_part, ctrl, fword, halo, face4, metric4};
         for (void* b : bufs) pool_free(b);
         pool_release_stream(stream);
     }
@@ -367,18 +369,29 @@ class SolverT : public Solver {
             for (int i = 0; i < hcap; i++)
                 hl[(size_t)t * hcap + i] = i < (int)halos[t].size() ? halos[t][i] : t * tile_elems;
         HIPST(hipStreamSynchronize(stream));  // queued launches may still read the old lists
-        pool_free(fword);
-        pool_free(halo);
+        for (void* b : {(void*)fword, (void*)halo, (void*)face4, (void*)metric4}) pool_free(b);
         fword = nullptr;
         halo = nullptr;
-        Status sa = pool_alloc((void**)&fword, words.size() * sizeof(uint16_t));
-        if (sa) sa = pool_alloc((void**)&halo, hl.size() * sizeof(int));
+        face4 = nullptr;
+        metric4 = nullptr;
+        tile_bytes = 0;
+        const size_t bw = words.size() * sizeof(uint16_t), bh = hl.size() * sizeof(int);
+        const size_t bf = (size_t)Kp * 3 * sizeof(float4), bm = (size_t)Kp * sizeof(float4);
+        Status sa = pool_alloc((void**)&fword, bw);
+        if (sa) sa = pool_alloc((void**)&halo, bh);
+        if (sa) sa = pool_alloc((void**)&face4, bf);
+        if (sa) sa = pool_alloc((void**)&metric4, bm);
         if (!sa) return sa;
-        HIPST(hipMemcpyAsync(fword, words.data(), words.size() * sizeof(uint16_t), hipMemcpyHostToDevice, stream));
-        HIPST(hipMemcpyAsync(halo, hl.data(), hl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        tile_bytes = (int64_t)(bw + bh + bf + bm);
+        HIPST(hipMemcpyAsync(fword, words.data(), bw, hipMemcpyHostToDevice, stream));
+        HIPST(hipMemcpyAsync(halo, hl.data(), bh, hipMemcpyHostToDevice, stream));
+        if constexpr (sizeof(R) == 4)
+            HIPST(launch_mfma_pack((const float*)geo_e, (const float*)geo_f, fword, K, Kp, face4, metric4, stream));
         HIPST(hipStreamSynchronize(stream));
         mx.fword = fword;
         mx.halo = halo;
+        mx.face4 = face4;
+        mx.metric4 = metric4;
         mx.hcap = hcap;
         return Status::ok();
     }
@@ -870,7 +883,7 @@ class SolverT : public Solver {
         return push_ctrl();
     }
     int kernel_kind() const override { return shape.kind; }
-    int64_t device_bytes() const override { return bytes; }
+    int64_t device_bytes() const override { return bytes + tile_bytes; }
     Status sync() override {
         HIPST(hipStreamSynchronize(stream));
         return Status::ok();
