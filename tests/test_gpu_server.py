@@ -71,6 +71,29 @@ def test_server_results_equal_alone(solo_min_elems):
         assert np.all(np.abs(m1 - m0) <= 2e-6 * np.abs(m0) + 1e-9)
 
 
+def test_server_solo_turnover():
+    """One slot, every solve launched with its own kernels: each solve is
+    admitted after the previous one retired and was freed, so the step graph
+    must be captured again for each (a new solve can be allocated where a
+    deleted one was).  Every result equals the request served alone."""
+    reqs = requests(6, seed=11)
+    sv = dg.Server(dg.DG_FP32, max_slots=1, steps_per_round=4, cfl=0.3, solo_min_elems=1, prep_threads=1)
+    sv.keep_states(True)
+    t0 = sv.now()
+    ids = []
+    for r in reqs:
+        r.arrival_s += t0
+        ids.append(sv.submit(r))
+    sv.drain()
+    res = {r.id: r for r in sv.poll(64)}
+    assert len(res) == len(reqs)
+    for k, r in enumerate(reqs):
+        assert res[ids[k]].status == 0
+        got = sv.get_state(ids[k], (4, dg.n_modes(r.p), 2 * r.nx * r.ny))
+        assert np.array_equal(got, alone(r, 1)), k
+    sv.close()
+
+
 def test_server_budget_admission():
     reqs = requests(8, seed=4)
     one = 40 * 2 * 24 * 24 * 4 * 15 * 4 * 2  # room for about two of the largest at a time
