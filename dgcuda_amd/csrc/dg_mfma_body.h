@@ -327,8 +327,9 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
     for (int v = 0; v < N; v++) {
         floatx4 c = {0.f, 0.f, 0.f, 0.f}, aa = {0.f, 0.f, 0.f, 0.f}, o, an;
         if constexpr (RK::C_FROM_INPUT) c = st.ub[v];
-        if constexpr (RK::LOAD_C) c = gp4(a.c)[v * VS + l4];
-        if constexpr (RK::LOAD_ACC) aa = gp4(a.acc)[v * VS + l4];
+        // single-use streams: non-temporal, as in the pipelined body
+        if constexpr (RK::LOAD_C) c = __builtin_nontemporal_load(&gp4(a.c)[v * VS + l4]);
+        if constexpr (RK::LOAD_ACC) aa = __builtin_nontemporal_load(&gp4(a.acc)[v * VS + l4]);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             float t = 0.f;
@@ -337,8 +338,8 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
         }
         // padding elements (e >= K) compute a copy of element K-1 into their own slots
         if (tile_groups(NP) == 4 || g < tile_groups(NP)) {
-            if constexpr (rk_writes_acc<KIND>()) gp4(a.acc)[v * VS + s4] = an;
-            gp4(a.out)[v * VS + s4] = o;
+            if constexpr (rk_writes_acc<KIND>()) __builtin_nontemporal_store(an, &gp4(a.acc)[v * VS + s4]);
+            __builtin_nontemporal_store(o, &gp4(a.out)[v * VS + s4]);
         }
         c0[v] = o[0];
     }
@@ -463,7 +464,12 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
     }
     // RK registers of this tile, then the next tile's state (st is dead
     // after this block).  For a stage whose input is the RK base state the
-    // operand registers already hold it in the output layout.
+    // operand registers already hold it in the output layout.  c and acc are
+    // read once per launch and the outputs are not read again in it: their
+    // accesses carry the non-temporal hint, which keeps the 4 MB L2 of an
+    // XCD for the stage input, whose lines neighbouring tiles read again as
+    // halo (measured: 0.135 -> 0.122 ms per stage at 524,288 triangles, p = 4;
+    // the hint on the geometry loads measured slower).
     constexpr int VS = tile_var_stride<P>();
     const unsigned l4 = tile_lane4<N, P>(ee, g);
     floatx4 cr[N], ar[N];
@@ -472,8 +478,8 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
         cr[v] = floatx4{0.f, 0.f, 0.f, 0.f};
         ar[v] = floatx4{0.f, 0.f, 0.f, 0.f};
         if constexpr (RK::C_FROM_INPUT) cr[v] = st.ub[v];
-        if constexpr (RK::LOAD_C) cr[v] = gp4(a.c)[v * VS + l4];
-        if constexpr (RK::LOAD_ACC) ar[v] = gp4(a.acc)[v * VS + l4];
+        if constexpr (RK::LOAD_C) cr[v] = __builtin_nontemporal_load(&gp4(a.c)[v * VS + l4]);
+        if constexpr (RK::LOAD_ACC) ar[v] = __builtin_nontemporal_load(&gp4(a.acc)[v * VS + l4]);
     }
     __builtin_amdgcn_sched_barrier(0);
     st.load_data(a.u, K, next, w, g, j, hid);
@@ -559,8 +565,8 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
         }
         // padding elements (e >= K) compute a copy of element K-1 into their own slots
         if (tile_groups(NP) == 4 || g < tile_groups(NP)) {
-            if constexpr (rk_writes_acc<KIND>()) gp4(a.acc)[v * VS + s4] = an;
-            gp4(a.out)[v * VS + s4] = o;
+            if constexpr (rk_writes_acc<KIND>()) __builtin_nontemporal_store(an, &gp4(a.acc)[v * VS + s4]);
+            __builtin_nontemporal_store(o, &gp4(a.out)[v * VS + s4]);
         }
         c0[v] = o[0];
     }
