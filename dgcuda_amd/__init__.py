@@ -105,7 +105,8 @@ def lib():
             "dg_integrate": (I, [VP, D, D, I, P(I64), dp]), "dg_run_steps": (I, [VP, I, D, I, I]),
             "dg_audit": (I, [VP, P(AuditOut)]), "dg_eval_vertices": (I, [VP, I, dp]),
             "dg_solver_sync": (I, [VP]), "dg_solver_set_kernel": (I, [VP, I]), "dg_solver_get_kernel": (I, [VP, ip]),
-            "dg_time_stage_kernels": (I, [VP, I, D, dp]), "dg_solver_device_bytes": (I, [VP, P(I64)]),
+            "dg_time_stage_kernels": (I, [VP, I, D, dp]), "dg_time_step_graph": (I, [VP, I, D, dp]),
+            "dg_solver_device_bytes": (I, [VP, P(I64)]),
             "dg_bandwidth_probe": (I, [I64, I, dp]), "dg_selftest_mfma": (I, [dp]),
             "dg_batch_create": (I, [I, P(VP)]), "dg_batch_destroy": (None, [VP]),
             "dg_batch_add": (I, [VP, VP, P(ProblemDesc), I, D, ip]), "dg_batch_project_ic": (I, [VP]),
@@ -335,6 +336,11 @@ class Solver:
         out = np.zeros(4)
         _check(lib().dg_time_stage_kernels(self._h, nsteps, cfl, _dp(out)))
         return dict(stage_ms=out[0], dt_ms=out[1], total_ms=out[2], launches=int(out[3]))
+
+    def time_step_graph(self, nsteps, cfl):
+        out = np.zeros(2)
+        _check(lib().dg_time_step_graph(self._h, nsteps, cfl, _dp(out)))
+        return dict(step_ms=out[0], total_ms=out[1])
 
     def device_bytes(self):
         b = C.c_int64()

@@ -318,6 +318,12 @@ int dg_time_stage_kernels(dg_solver* s, int nsteps, double cfl, double* out) {
     GUARD_BEGIN return ret(s->s->time_stage_kernels(nsteps, cfl, out)); GUARD_END
 }
 
+int dg_time_step_graph(dg_solver* s, int nsteps, double cfl, double* out) {
+    SOLVER_CHECK(s);
+    if (nsteps <= 0 || !out) return fail(DG_ERR_ARG, "invalid argument");
+    GUARD_BEGIN return ret(s->s->time_step_graph(nsteps, cfl, out)); GUARD_END
+}
+
 int dg_solver_device_bytes(const dg_solver* s, int64_t* bytes) {
     SOLVER_CHECK(s);
     if (bytes) *bytes = s->s->device_bytes();

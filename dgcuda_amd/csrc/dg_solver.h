@@ -109,6 +109,8 @@ class Solver {
     virtual Status audit(dg_audit_out* out) = 0;
     virtual Status eval_vertices(int var, double* uv) = 0;
     virtual Status time_stage_kernels(int nsteps, double cfl, double* out) = 0;
+    // nsteps graph-replayed steps between one event pair: out[0] ms/step, out[1] total ms
+    virtual Status time_step_graph(int nsteps, double cfl, double* out) = 0;
     virtual Status set_kernel(int kind) = 0;
     virtual int kernel_kind() const = 0;
     virtual int64_t device_bytes() const = 0;

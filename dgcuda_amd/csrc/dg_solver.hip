@@ -791,6 +791,31 @@ class SolverT : public Solver {
         return Status::ok();
     }
 
+    Status time_step_graph(int nsteps, double cfl, double* out) override {
+        Status st = begin_run(0.0, false, cfl);
+        if (!st) return st;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPST(hipEventCreate(&e0));
+        HIPST(hipEventCreate(&e1));
+        HIPST(hipEventRecord(e0, stream));
+        for (int i = 0; i < nsteps && st; i++) st = enqueue_cfl_step(DG_RK4, true);
+        float ms = 0.f;
+        hipError_t e = hipEventRecord(e1, stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        if (!st) return st;
+        HIPST(e);
+        out[0] = ms / nsteps;
+        out[1] = ms;
+        if (!(st = pull_ctrl())) return st;
+        t = hctrl.t;
+        steps = hctrl.steps;
+        lam_valid = true;
+        return Status::ok();
+    }
+
     Status time_stage_kernels(int nsteps, double cfl, double* out) override {
         Status st = begin_run(0.0, false, cfl);
         if (!st) return st;

@@ -174,6 +174,13 @@ int dg_solver_get_kernel(const dg_solver* s, int* kind);
  * out[1] = mean dt-kernel ms, out[2] = total ms, out[3] = number of stage
  * launches timed. */
 int dg_time_stage_kernels(dg_solver* s, int nsteps, double cfl, double* out);
+/* Runs nsteps RK4 CFL steps replayed from the captured step graph (the
+ * form dg_integrate runs) with one HIP event pair on the solver's stream
+ * around all of them: out[0] = ms per step, out[1] = total ms.  Warm up
+ * first (e.g. dg_solver_run_steps with the graph): a graph not yet captured
+ * is captured inside the timed span.  Measurement aid; no counterpart in
+ * the reference. */
+int dg_time_step_graph(dg_solver* s, int nsteps, double cfl, double* out);
 /* device buffer bytes held by the solver */
 int dg_solver_device_bytes(const dg_solver* s, int64_t* bytes);
 /* Device-to-device streaming copy of `bytes` (multiple of 16), iters times,
