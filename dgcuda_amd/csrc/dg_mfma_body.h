@@ -181,6 +181,23 @@ __device__ __forceinline__ float rk_update(float c, float aa, float k, float* ac
         return c + k;
     }
 }
+// Cell wave speed |u| + c of the new state's cell mean for the CFL step
+// (euler.cu:252-278, in the bulk precision as the reference computes it):
+// FP32 with the hardware reciprocal and square root, as the fluxes.
+template <int N>
+__device__ __forceinline__ float cell_speed_f32(const PhysP& ph, const float* U) {
+    if constexpr (N == 1) {
+        const float ax = (float)ph.ax, ay = (float)ph.ay;
+        return __builtin_amdgcn_sqrtf(ax * ax + ay * ay);
+    } else {
+        const float g = (float)ph.gamma;
+        const float ir = __builtin_amdgcn_rcpf(U[0]);
+        const float m2 = U[1] * U[1] + U[2] * U[2];
+        const float pr = (g - 1.f) * (U[3] - 0.5f * m2 * ir);
+        return __builtin_amdgcn_sqrtf(m2) * ir + __builtin_amdgcn_sqrtf(g * pr * ir);
+    }
+}
+
 template <int KIND>
 constexpr bool rk_writes_acc() {
     return KIND == ST_RK4_1 || KIND == ST_RK4_2 || KIND == ST_RK4_3;
@@ -345,11 +362,11 @@ __device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, c
     }
     if constexpr (LAST) {
         if (g == 0 && valid) {
-            double Uc[4];
-            const double b0 = tab<double, P>().B[0][0];
+            float Uc[4];
+            const float b0 = tab<float, P>().B[0][0];
 #pragma unroll
-            for (int v = 0; v < N; v++) Uc[v] = (double)c0[v] * b0;
-            lam = cell_speed<N>(a.phys, Uc);
+            for (int v = 0; v < N; v++) Uc[v] = c0[v] * b0;
+            lam = (double)cell_speed_f32<N>(a.phys, Uc);
         }
     }
     __syncthreads();  // the next tile reuses the LDS
@@ -572,11 +589,11 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<float>&
     }
     if constexpr (LAST) {
         if (g == 0 && valid) {
-            double Uc[4];
-            const double b0 = tab<double, P>().B[0][0];
+            float Uc[4];
+            const float b0 = tab<float, P>().B[0][0];
 #pragma unroll
-            for (int v = 0; v < N; v++) Uc[v] = (double)c0[v] * b0;
-            lam = cell_speed<N>(a.phys, Uc);
+            for (int v = 0; v < N; v++) Uc[v] = c0[v] * b0;
+            lam = (double)cell_speed_f32<N>(a.phys, Uc);
         }
     }
     __syncthreads();  // the next tile reuses the LDS
