@@ -30,6 +30,8 @@ struct SlotDesc {
     const int* nbr;
     const uint16_t* fword;
     const int* halo;
+    const float4* face4;
+    const float4* metric4;
     Ctrl* ctrl;
     int K, Kp, N, p, hcap, path;  // path 1 = lane per element, 2 = matrix-core tiles
     PhysP phys;
@@ -56,33 +58,47 @@ __device__ __forceinline__ T* uni(T* ptr) {
     return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
 }
 
-// One wavefront-group's run of consecutive items of the same solve
-// (items it, it+G8, ... while they belong to `slot`), with the next tile's
-// state in flight while the current one computes, as in the single-solve
-// kernel.  Returns the first item index not processed.
+// Tile of item `x` if it belongs to `slot`, else `fallback` (a tile of the
+// same solve, so loads issued for it stay in bounds).
+__device__ __forceinline__ int run_tile(const int2* __restrict__ items, int x, int i1, int slot, int fallback) {
+    if (x >= i1) return fallback;
+    const int2 v = items[x];
+    return uni(v.x) == slot ? uni(v.y) : fallback;
+}
+
+// One workgroup's run of consecutive items of the same solve (items it,
+// it+G8, ... while they belong to `slot`) through the pipelined tile body of
+// the single-solve kernel (the same arithmetic, so a batched solve equals
+// the solve run alone bit for bit): the state and geometry of the next tile
+// and the halo ids of the one after are in flight while a tile computes.
+// Returns the first item index not processed.
 template <int N, int P, int KIND>
 __device__ __forceinline__ int batch_mfma_run(const StageArgs<float>& a, const MfmaArgs& mx, int slot,
                                               const int2* __restrict__ items, int it, int i1, int G8, float4* fd,
-                                              float* ts, double& lam) {
+                                              float* ts, float4* tabA, double& lam) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
-    MfmaA<N, P> A;
-    A.init(g, j, it >> 31);
-    MfmaTileState<N, P> cur, nxt;
+    const float dt = (float)uni(a.ctrl->dt);
     int tile = uni(items[it].y);
-    cur.load(a.u, a.K, mx.halo, mx.hcap, tile, w, g, j);
+    int n1 = run_tile(items, it + G8, i1, slot, tile);
+    MfmaTileState<N, P> st;
+    int hid[MfmaTileState<N, P>::HPW];
+    TileGeo gc;
+    MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, tile, w, j, hid);
+    gc.load(a, mx, tile, w * 16 + j);
+    st.load_data(a.u, a.K, tile, w, g, j, hid);
+    MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, n1, w, j, hid);
+    if (w == 0) mfma_a_to_lds<N, P>(tabA, lane);  // the previous run ended with a barrier
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
     while (true) {
         const int nit = it + G8;
-        int ntile = -1;
-        if (nit < i1) {
-            const int2 ni = items[nit];
-            if (uni(ni.x) == slot) ntile = uni(ni.y);
-        }
-        if (ntile >= 0) nxt.load(a.u, a.K, mx.halo, mx.hcap, ntile, w, g, j);
-        lam = fmax(lam, mfma_process_tile<N, P, KIND>(a, mx, A, cur, tile, fd, ts));
-        if (ntile < 0) return nit;
+        const bool more = nit < i1 && uni(items[nit].x) == slot;
+        const int n2 = run_tile(items, it + 2 * G8, i1, slot, n1);
+        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND>(a, mx, tabA, st, hid, gc, dt, tile, n1, n2, fd, ts));
+        if (!more) return nit;
         it = nit;
-        tile = ntile;
-        cur = nxt;
+        tile = n1;
+        n1 = n2;
     }
 }
 
@@ -185,6 +201,7 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float
     __shared__ double sh[8];
     float4* fd = lds4;
     float* ts = reinterpret_cast<float*>(lds4 + MFMA_TILE * 3);
+    __shared__ float4 tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
     const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
     batch_lane_items<float, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
 
@@ -201,17 +218,17 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float
         MfmaArgs mx;
         mx.fword = uni(sd->fword);
         mx.halo = uni(sd->halo);
-        mx.face4 = nullptr;  // the batch body reads geo_f and fword
-        mx.metric4 = nullptr;
+        mx.face4 = uni(sd->face4);
+        mx.metric4 = uni(sd->metric4);
         mx.hcap = uni(sd->hcap);
         mx.pad = 0;
         const int N = uni(sd->N), p = uni(sd->p);
         double lam = 0.0;
         int next = it + G8;
-        if (N == 4 && p == 4) next = batch_mfma_run<4, 4, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, lam);
-        else if (N == 4 && p == 3) next = batch_mfma_run<4, 3, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, lam);
-        else if (N == 1 && p == 4) next = batch_mfma_run<1, 4, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, lam);
-        else if (N == 1 && p == 3) next = batch_mfma_run<1, 3, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, lam);
+        if (N == 4 && p == 4) next = batch_mfma_run<4, 4, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        else if (N == 4 && p == 3) next = batch_mfma_run<4, 3, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        else if (N == 1 && p == 4) next = batch_mfma_run<1, 4, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        else if (N == 1 && p == 3) next = batch_mfma_run<1, 3, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
         it = next;
         if constexpr (LAST) {
             const double m = block_max(lam, sh);
@@ -374,6 +391,8 @@ class EngineT : public BatchEngine {
             d.nbr = si.nbr;
             d.fword = si.fword;
             d.halo = si.halo;
+            d.face4 = si.face4;
+            d.metric4 = si.metric4;
             d.ctrl = si.ctrl;
             d.K = si.K;
             d.Kp = si.Kp;

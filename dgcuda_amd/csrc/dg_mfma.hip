@@ -4,8 +4,8 @@
 // Persistent: each workgroup walks a strided sequence of 64-element tiles
 // inside the contiguous tile range of its XCD (so Morton neighbours share
 // that XCD's L2), and the state of its next tile is in flight while it
-// computes the current one.  The per-tile work is mfma_process_tile
-// (dg_mfma_body.h).
+// computes the current one.  The per-tile work is mfma_process_tile_pipe
+// (dg_mfma_body.h), shared with the batched kernel.
 #include "dg_mfma_body.h"
 
 namespace dgmi {

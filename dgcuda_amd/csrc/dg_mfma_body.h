@@ -203,176 +203,6 @@ constexpr bool rk_writes_acc() {
     return KIND == ST_RK4_1 || KIND == ST_RK4_2 || KIND == ST_RK4_3;
 }
 
-// Processes one tile (all 4 wavefronts of the workgroup call it).  Returns
-// the lane's maximum cell wave speed of the new state for the last stage
-// of a step (0 otherwise).  Contains two workgroup barriers.
-template <int N, int P, int KIND>
-__device__ __forceinline__ double mfma_process_tile(const StageArgs<float>& a, const MfmaArgs& mx,
-                                                    const MfmaA<N, P>& A, const MfmaTileState<N, P>& st, int tile,
-                                                    float4* fd, float* ts) {
-    constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1, NPT = 3 * NQ1;
-    constexpr int TILE = MFMA_TILE;
-    constexpr int TS = 17 * N;
-    constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
-    constexpr int HPW = MfmaTileState<N, P>::HPW;
-    static_assert(NP <= 16 && NQ <= 16 && NPT <= 16, "order too high for 16x16 tiles");
-    const int K = a.K, Kp = a.Kp;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int g = lane >> 4, j = lane & 15;
-    const int npairs = (mx.hcap >> 4) * N;
-    const float dt = (float)a.ctrl->dt;
-    const int base = tile * TILE;
-    const int el = w * 16 + j;
-    const int e = base + el;
-    const bool valid = e < K;
-    const int ee = valid ? e : K - 1;
-
-    for (int i = threadIdx.x; i < TILE * 3; i += blockDim.x) {
-        const int l = i / 3, f = i - 3 * l;
-        const unsigned e2 = (unsigned)min(base + l, K - 1);
-        const DG_GLOBAL float* gf = gp(a.geo_f) + (size_t)(3 * f) * Kp;
-        float4 r;
-        r.x = gf[e2];
-        r.y = (gf + Kp)[e2];
-        r.z = (gf + 2 * (size_t)Kp)[e2];
-        r.w = __int_as_float((int)(gp(mx.fword) + (size_t)f * Kp)[e2]);
-        fd[i] = r;
-    }
-    const unsigned ueu = (unsigned)ee;
-    const DG_GLOBAL float* ge = gp(a.geo_e);
-    const float xr = ge[ueu], yr = (ge + Kp)[ueu], xs = (ge + 2 * (size_t)Kp)[ueu];
-    const float ys = (ge + 3 * (size_t)Kp)[ueu], Jv = (ge + 4 * (size_t)Kp)[ueu];
-
-    // ---- phase 1: face traces of the own sub-tile and of this wave's halo pairs ----
-#pragma unroll
-    for (int v = 0; v < N; v++) {
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 4; s++) acc = mfma4(A.tr[s], st.ub[v][s], acc);
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (4 * g + i < NPT) ts[el * TS + (4 * g + i) * N + v] = acc[i];
-    }
-#pragma unroll
-    for (int q = 0; q < HPW; q++) {
-        const int pr = w + 4 * q;
-        if (pr < npairs) {
-            const int slot = TILE + (pr / N) * 16 + j, v = pr % N;
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int s = 0; s < 4; s++) acc = mfma4(A.tr[s], st.hb[q][s], acc);
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (4 * g + i < NPT) ts[slot * TS + (4 * g + i) * N + v] = acc[i];
-        }
-    }
-    __syncthreads();
-
-    // ---- phase 2: volume, faces, inverse mass, RK update ----
-    floatx4 U[N];
-#pragma unroll
-    for (int v = 0; v < N; v++) {
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 4; s++) acc = mfma4(A.vol[s], st.ub[v][s], acc);
-        U[v] = acc;
-    }
-    floatx4 R[N];
-#pragma unroll
-    for (int v = 0; v < N; v++) R[v] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        // volume point q = 4g+i: flux, metric, reduction step i
-        float Uq[N], fx[N], fy[N];
-#pragma unroll
-        for (int v = 0; v < N; v++) Uq[v] = U[v][i];
-        const bool qv = (4 * g + i) < NQ;
-        if (N == 4 && !qv) Uq[0] = 1.f;  // padding point: keep the flux finite
-        flux<N, float>(a.phys, Uq, fx, fy);
-#pragma unroll
-        for (int v = 0; v < N; v++) {
-            const float fr = qv ? fx[v] * ys - fy[v] * xs : 0.f;
-            const float fs = qv ? fy[v] * xr - fx[v] * yr : 0.f;
-            R[v] = mfma4(A.gx[i], fr, R[v]);
-            R[v] = mfma4(A.gy[i], fs, R[v]);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        // own face point pt = 4g+i: LLF flux in the side's orientation
-        const int pt = 4 * g + i;
-        float G[N];
-        if (pt < NPT) {
-            const int f = pt / NQ1, jo = pt - f * NQ1;
-            const float4 r = fd[el * 3 + f];
-            const int wd = __float_as_int(r.w);
-            const int slot = wd & 0xFFF, fn = (wd >> 12) & 3;
-            const bool left = (wd >> 14) & 1;
-            const float* own = ts + el * TS + pt * N;
-            const float* nbt = ts + slot * TS + (fn * NQ1 + NQ1 - 1 - jo) * N;
-            float UL[N], UR[N];
-#pragma unroll
-            for (int v = 0; v < N; v++) {
-                const float uo = own[v], un = nbt[v];
-                UL[v] = left ? uo : un;
-                UR[v] = left ? un : uo;
-            }
-            float fxl[N], fyl[N], fxr[N], fyr[N];
-            flux<N, float>(a.phys, UL, fxl, fyl);
-            flux<N, float>(a.phys, UR, fxr, fyr);
-            const float lm = llf_speed<N, float>(a.phys, UL, UR, r.x, r.y);
-            const float coef = left ? -0.5f * r.z : 0.5f * r.z;
-#pragma unroll
-            for (int v = 0; v < N; v++)
-                G[v] = coef * (0.5f * ((fxl[v] + fxr[v]) * r.x + (fyl[v] + fyr[v]) * r.y + lm * (UL[v] - UR[v])));
-        } else {
-#pragma unroll
-            for (int v = 0; v < N; v++) G[v] = 0.f;
-        }
-#pragma unroll
-        for (int v = 0; v < N; v++) R[v] = mfma4(A.sp[i], G[v], R[v]);
-    }
-
-    // inverse mass and the RK stage update; lane holds modes m = 4g+i
-    using RK = RkNeeds<KIND>;
-    constexpr int VS = tile_var_stride<P>();
-    const unsigned l4 = tile_lane4<N, P>(ee, g), s4 = tile_lane4<N, P>(e, g);
-    const float cj = __builtin_amdgcn_rcpf(Jv) * dt;
-    float c0[N];
-    double lam = 0.0;
-#pragma unroll
-    for (int v = 0; v < N; v++) {
-        floatx4 c = {0.f, 0.f, 0.f, 0.f}, aa = {0.f, 0.f, 0.f, 0.f}, o, an;
-        if constexpr (RK::C_FROM_INPUT) c = st.ub[v];
-        // single-use streams: non-temporal, as in the pipelined body
-        if constexpr (RK::LOAD_C) c = __builtin_nontemporal_load(&gp4(a.c)[v * VS + l4]);
-        if constexpr (RK::LOAD_ACC) aa = __builtin_nontemporal_load(&gp4(a.acc)[v * VS + l4]);
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            float t = 0.f;
-            o[i] = rk_update<KIND>(c[i], aa[i], cj * R[v][i], &t);
-            an[i] = t;
-        }
-        // padding elements (e >= K) compute a copy of element K-1 into their own slots
-        if (tile_groups(NP) == 4 || g < tile_groups(NP)) {
-            if constexpr (rk_writes_acc<KIND>()) __builtin_nontemporal_store(an, &gp4(a.acc)[v * VS + s4]);
-            __builtin_nontemporal_store(o, &gp4(a.out)[v * VS + s4]);
-        }
-        c0[v] = o[0];
-    }
-    if constexpr (LAST) {
-        if (g == 0 && valid) {
-            float Uc[4];
-            const float b0 = tab<float, P>().B[0][0];
-#pragma unroll
-            for (int v = 0; v < N; v++) Uc[v] = c0[v] * b0;
-            lam = (double)cell_speed_f32<N>(a.phys, Uc);
-        }
-    }
-    __syncthreads();  // the next tile reuses the LDS
-    return lam;
-}
-
 // Geometry of one tile from the packed arrays: the (face normal, length,
 // packed neighbour word) record this thread stores to LDS (threads 0..191
 // hold one of the 64 x 3 records; the others load record 191 again and

@@ -874,6 +874,8 @@ class SolverT : public Solver {
         o->nbr = nbr;
         o->fword = fword;
         o->halo = halo;
+        o->face4 = face4;
+        o->metric4 = metric4;
         o->ctrl = ctrl;
         o->K = K;
         o->Kp = Kp;
