@@ -111,3 +111,28 @@ def test_kernel_switch_carries_state(p):
     assert np.array_equal(got, b.get_state())
     s.set_kernel(1)
     assert np.array_equal(s.get_state(), got)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_tiny_meshes_every_kernel(n):
+    """Meshes of 2 to 50 triangles: one partial tile, neighbours reached
+    across several periodic sides, halo lists of tile elements only.  Every
+    kernel, at every order the matrix-core kernels take (p = 1, 2 use a
+    single mode group of the tile layout), matches the oracle's RHS: FP64
+    lane kernel to 1e-12, FP32 kernels to 2e-5 (relative max-norm)."""
+    m = dg.Mesh.periodic_jittered(n, n, 0.0, 10.0, 0.0, 10.0, 0.1, 7)
+    for N in (1, 4):
+        prob = problem_for(N)
+        for p in (1, 2, 3, 4):
+            print("tiny mesh n=%d N=%d p=%d" % (n, N, p), flush=True)
+            o = orc.Oracle.from_mesh(m, p, prob)
+            u = perturbed_state(o, prob, m.period, seed=n + p)
+            ref = o.rhs(u, dt=0.01)
+            assert np.isfinite(ref).all(), "oracle reference not finite: test data out of the physical range"
+            s64 = dg.Solver(m, prob, p, dg.DG_FP64)
+            assert rel_max(s64.rhs(u, dt=0.01), ref) <= 1e-12
+            for kind in (1, 2, 3):
+                s = dg.Solver(m, prob, p, dg.DG_FP32)
+                s.set_kernel(kind)
+                err = rel_max(s.rhs(u, dt=0.01), ref)
+                assert err <= 2e-5, (kind, err)
