@@ -40,13 +40,14 @@ def algorithmic_bytes_per_elem(N, n_p, s, kernel):
     (stage 1 reads u_n and writes acc and the next input; stages 2-3 read
     the input, u_n and acc and write acc and the next input; stage 4 reads
     the input and acc, which carries u_n, and writes u_n+1), plus the packed
-    geometry: per face one 16-byte record (nx, ny, len, neighbour word), per
-    element a 16-byte map (xr, yr, xs, ys) and J, 68 bytes.  Lane-per-element
+    geometry: per face one record of 4 values (nx, ny, len, neighbour word),
+    per element a map of 4 values (xr, yr, xs, ys) and J, 17*s bytes (68 in
+    FP32).  Lane-per-element
     path (kernel 1): the reference's grouping, where stage 4 also reads u_n
     (4.25 passes), geometry 14*s (xr, yr, xs, ys, J; per face nx, ny, len)
     and 3 32-bit neighbour words."""
     if kernel >= 2:
-        return 4.0 * N * n_p * s + 3 * 16 + 16 + 4
+        return 4.0 * N * n_p * s + (3 * 4 + 4 + 1) * s
     return 4.25 * N * n_p * s + 14 * s + 3 * 4
 
 

@@ -30,8 +30,8 @@ struct SlotDesc {
     const int* nbr;
     const uint16_t* fword;
     const int* halo;
-    const float4* face4;
-    const float4* metric4;
+    const void* face4;
+    const void* metric4;
     Ctrl* ctrl;
     int K, Kp, N, p, hcap, path;  // path 1 = lane per element, 2 = matrix-core tiles
     PhysP phys;
@@ -74,27 +74,27 @@ __device__ __forceinline__ int run_tile(const int2* __restrict__ items, int x, i
 // Returns the first item index not processed.
 template <int N, int P, int KIND>
 __device__ __forceinline__ int batch_mfma_run(const StageArgs<float>& a, const MfmaArgs& mx, int slot,
-                                              const int2* __restrict__ items, int it, int i1, int G8, float4* fd,
-                                              float* ts, float4* tabA, double& lam) {
+                                              const int2* __restrict__ items, int it, int i1, int G8, floatx4* fd,
+                                              float* ts, floatx4* tabA, double& lam) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
     const float dt = (float)uni(a.ctrl->dt);
     int tile = uni(items[it].y);
     int n1 = run_tile(items, it + G8, i1, slot, tile);
-    MfmaTileState<N, P> st;
-    int hid[MfmaTileState<N, P>::HPW];
-    TileGeo gc;
-    MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, tile, w, j, hid);
+    MfmaTileState<N, P, float> st;
+    int hid[MfmaTileState<N, P, float>::HPW];
+    TileGeo<float> gc;
+    MfmaTileState<N, P, float>::load_ids(mx.halo, mx.hcap, tile, w, j, hid);
     gc.load(a, mx, tile, w * 16 + j);
     st.load_data(a.u, a.K, tile, w, g, j, hid);
-    MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, n1, w, j, hid);
-    if (w == 0) mfma_a_to_lds<N, P>(tabA, lane);  // the previous run ended with a barrier
+    MfmaTileState<N, P, float>::load_ids(mx.halo, mx.hcap, n1, w, j, hid);
+    if (w == 0) mfma_a_to_lds<N, P, float>(tabA, lane);  // the previous run ended with a barrier
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     while (true) {
         const int nit = it + G8;
         const bool more = nit < i1 && uni(items[nit].x) == slot;
         const int n2 = run_tile(items, it + 2 * G8, i1, slot, n1);
-        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND>(a, mx, tabA, st, hid, gc, dt, tile, n1, n2, fd, ts));
+        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND, float>(a, mx, tabA, st, hid, gc, dt, tile, n1, n2, fd, ts));
         if (!more) return nit;
         it = nit;
         tile = n1;
@@ -199,9 +199,9 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float
     constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
     extern __shared__ float4 lds4[];
     __shared__ double sh[8];
-    float4* fd = lds4;
-    float* ts = reinterpret_cast<float*>(lds4 + MFMA_TILE * 3);
-    __shared__ float4 tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
+    floatx4* fd = reinterpret_cast<floatx4*>(lds4);
+    float* ts = reinterpret_cast<float*>(fd + MFMA_TILE * 3);
+    __shared__ floatx4 tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
     const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
     batch_lane_items<float, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
 
@@ -324,7 +324,7 @@ class EngineT : public BatchEngine {
         HIPST(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPST(hipMalloc((void**)&d_xcd, BATCH_PARTS * sizeof(int)));
         HIPST(hipMemsetAsync(d_xcd, 0, BATCH_PARTS * sizeof(int), st));
-        lds = (sizeof(R) == 4) ? mfma_tile_lds_bytes(4, MFMA_MAX_HALO) : 0;
+        lds = (sizeof(R) == 4) ? mfma_tile_lds_bytes(4, MFMA_MAX_HALO, 4) : 0;
         int per_cu = 0, dev = 0, cus = 0;
         HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BatchKernel<R, ST_RK4_2>::fn(), 256, lds));
         HIPST(hipGetDevice(&dev));

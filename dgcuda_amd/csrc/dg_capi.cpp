@@ -372,14 +372,17 @@ int dg_selftest_mfma(double* max_err) {
     Status st = ensure_device_ready();
     if (!st) return ret(st);
     float d[256];
+    double d64[256];
     hipError_t e = mfma_layout_probe(d);
+    if (e == hipSuccess) e = mfma_layout_probe_f64(d64);
     if (e != hipSuccess) return ret(hip_status(e, "mfma probe"));
-    double err = 0;
+    double err = 0;  // both products, as laid out by the kernels' lane maps
     for (int i = 0; i < 16; i++)
         for (int j = 0; j < 16; j++) {
             double want = 0;
             for (int k = 0; k < 4; k++) want += ((i + 1) + 0.5 * k) * ((2 * k + 1) - 0.25 * j);
             err = std::max(err, std::fabs(want - (double)d[i * 16 + j]));
+            err = std::max(err, std::fabs(want - d64[i * 16 + j]));
         }
     *max_err = err;
     return DG_OK;

@@ -152,24 +152,30 @@ enum StageKind {
 // Placement of entry (variable v, mode m, element e) of a state array in HBM.
 //  * Mode-major (the lane-per-element kernels): (v*NP + m)*Kp + e, so the
 //    lanes of a wavefront read consecutive elements of one mode.
-//  * Tile layout (FP32 state of the matrix-core kernels): per 16-element
-//    block and variable, G = ceil(NP/4) groups of [16 elements][4 modes],
-//    (((e/16)*N + v)*G + m/4)*64 + (e%16)*4 + m%4.  Lane j + 16g of a
-//    wavefront holds modes 4g..4g+3 of element j, one 16-byte access, and
-//    the wavefront's 64 lanes cover 1 KiB contiguous.  The modes n_p..4G-1
-//    of the last group are padding and hold zeros.
+//  * Tile layout of the matrix-core kernels: per 16-element block and
+//    variable, groups g of [16 elements][4 modes] with lane j + 16g of a
+//    wavefront holding its element's 4 modes in one access (the modes the
+//    lane's matrix products leave in its registers):
+//    - FP32 (tiled = 1): G = ceil(NP/4) groups, group g = modes 4g..4g+3,
+//      (((e/16)*N + v)*G + m/4)*64 + (e%16)*4 + m%4;
+//    - FP64 (tiled = 2): 4 groups, group g = modes g, g+4, g+8, g+12,
+//      ((e/16)*N + v)*256 + (m%4)*64 + (e%16)*4 + m/4.
+//    The wavefront's 64 lanes cover the block's variable contiguously.
+//    Modes >= NP are padding and hold zeros.
 constexpr int tile_groups(int NP) { return (NP + 3) / 4; }
 
 struct StateIx {
     int N, NP, Kp, tiled;
     __host__ __device__ size_t at(int v, int m, int e) const {
         if (!tiled) return ((size_t)v * NP + m) * Kp + e;
+        if (tiled == 2) return (((size_t)(e >> 4) * N + v) << 8) + (size_t)(((m & 3) << 6) + ((e & 15) << 2) + (m >> 2));
         const int G = tile_groups(NP);
         return ((((size_t)(e >> 4) * N + v) * G + (m >> 2)) << 6) + (size_t)((e & 15) << 2) + (m & 3);
     }
-    // entries of one array: N*NP*Kp, or N*4G*Kp with the padding modes
+    // entries of one array: N*NP*Kp, or the tile layout's padded count
     __host__ __device__ size_t elems() const {
-        return (size_t)N * (tiled ? 4 * tile_groups(NP) : NP) * Kp;
+        if (!tiled) return (size_t)N * NP * Kp;
+        return (size_t)N * (tiled == 2 ? 16 : 4 * tile_groups(NP)) * Kp;
     }
 };
 

@@ -29,31 +29,35 @@ hipError_t launch_stage_valu(int N, int p, const StageArgs<R>& a, dim3 grid, hip
 // 16-bit word {bits 0..11 LDS slot of the neighbour (own tile element or
 // halo entry), bits 12..13 neighbour's local face, bit 14 left element},
 // stored [3][Kp]; per tile the halo element list [ntiles][hcap]; and the
-// packed geometry of the single-solve kernel: per (element, face) one
-// float4 (nx, ny, len, word as bits) [Kp][3], per element one float4
-// (xr, yr, xs, ys) [Kp], padding elements replicating element K-1.
+// packed geometry in the bulk precision: per (element, face) 4 values
+// (nx, ny, len, word) [Kp][3], per element 4 values (xr, yr, xs, ys) [Kp],
+// padding elements replicating element K-1.
 struct MfmaArgs {
     const uint16_t* fword;
     const int* halo;
-    const float4* face4;
-    const float4* metric4;
+    const void* face4;
+    const void* metric4;
     int hcap;
     int pad;
 };
-hipError_t launch_mfma_pack(const float* geo_e, const float* geo_f, const uint16_t* fword, int K, int Kp,
-                            float4* face4, float4* metric4, hipStream_t st);
+template <typename R>
+hipError_t launch_mfma_pack(const R* geo_e, const R* geo_f, const uint16_t* fword, int K, int Kp, void* face4,
+                            void* metric4, hipStream_t st);
 
-// matrix-core element kernel (FP32, p = 1..4, state in the tile layout):
-// 64-element tiles, 4 wavefronts per workgroup, persistent over the tiles
-// of its XCD's range when the grid is smaller than the tile count; returns
-// hipErrorInvalidValue for unsupported (N, p)
+// matrix-core element kernel (FP32 or FP64, p = 1..4, state in the tile
+// layout): 64-element tiles, 4 wavefronts per workgroup, persistent over the
+// tiles of its XCD's range when the grid is smaller than the tile count;
+// returns hipErrorInvalidValue for unsupported (N, p)
 constexpr int MFMA_TILE_ELEMS = 64;
 constexpr int MFMA_MAX_HALO = 64;
-hipError_t launch_stage_mfma(int N, int p, const StageArgs<float>& a, const MfmaArgs& mx, dim3 grid, hipStream_t st);
+template <typename R>
+hipError_t launch_stage_mfma(int N, int p, const StageArgs<R>& a, const MfmaArgs& mx, dim3 grid, hipStream_t st);
 bool mfma_supported(int N, int p);
-size_t mfma_lds_bytes(int N, int hcap);
-int mfma_resident_groups(int N, int p, int hcap);
+// dynamic LDS of a launch, `rs` = bytes per value of the bulk precision
+size_t mfma_lds_bytes(int N, int hcap, int rs);
+int mfma_resident_groups(int N, int p, int hcap, int rs);
 hipError_t mfma_layout_probe(float* host_out);
+hipError_t mfma_layout_probe_f64(double* host_out);
 constexpr int VALU_TILE_ELEMS = 128;
 
 // resets ctrl->lam_bits and accumulates the state's maximum cell wave speed into it

@@ -62,8 +62,8 @@ struct SlotInfo {
     const int* nbr;
     const uint16_t* fword;
     const int* halo;
-    const float4* face4;    // packed geometry of the matrix-core kernels
-    const float4* metric4;
+    const void* face4;      // packed geometry of the matrix-core kernels
+    const void* metric4;
     Ctrl* ctrl;
     int K, Kp, N, p, hcap, kind;
     PhysP phys;

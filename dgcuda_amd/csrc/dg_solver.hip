@@ -212,7 +212,7 @@ class SolverT : public Solver {
     std::vector<int> h_nbr[3];
     uint16_t* fword = nullptr;
     int* halo = nullptr;
-    float4 *face4 = nullptr, *metric4 = nullptr;
+    void *face4 = nullptr, *metric4 = nullptr;
     int64_t tile_bytes = 0;  // fword, halo and packed geometry
     MfmaArgs mx{};
     double *V = nullptr, *Jd = nullptr, *audit_part = nullptr;
@@ -239,9 +239,12 @@ class SolverT : public Solver {
         pool_release_stream(stream);
     }
 
+    // StateIx::tiled of this precision's tile layout
+    static constexpr int tile_kind() { return sizeof(R) == 4 ? 1 : 2; }
+
     // entries allocated per state array: room for either layout
     size_t state_alloc() const {
-        StateIx t{N, NP, Kp, sizeof(R) == 4 && mfma_supported(N, p) ? 1 : 0};
+        StateIx t{N, NP, Kp, mfma_supported(N, p) ? tile_kind() : 0};
         return t.elems();
     }
 
@@ -362,7 +365,8 @@ class SolverT : public Solver {
             hmax = std::max(hmax, (int)h.size());
         }
         const int hcap = std::max(16, (hmax + 15) / 16 * 16);
-        if (hcap > MFMA_MAX_HALO || mfma_lds_bytes(N, hcap) > 65536)
+        // LDS per workgroup: traces and face records, plus the 5-slot operand table
+        if (hcap > MFMA_MAX_HALO || mfma_lds_bytes(N, hcap, (int)sizeof(R)) + 5 * 64 * 4 * sizeof(R) > 160 * 1024)
             return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large for the matrix-core kernel");
         std::vector<int> hl((size_t)ntiles * hcap);
         for (int t = 0; t < ntiles; t++)
@@ -376,7 +380,7 @@ class SolverT : public Solver {
         metric4 = nullptr;
         tile_bytes = 0;
         const size_t bw = words.size() * sizeof(uint16_t), bh = hl.size() * sizeof(int);
-        const size_t bf = (size_t)Kp * 3 * sizeof(float4), bm = (size_t)Kp * sizeof(float4);
+        const size_t bf = (size_t)Kp * 3 * 4 * sizeof(R), bm = (size_t)Kp * 4 * sizeof(R);
         Status sa = pool_alloc((void**)&fword, bw);
         if (sa) sa = pool_alloc((void**)&halo, bh);
         if (sa) sa = pool_alloc((void**)&face4, bf);
@@ -385,8 +389,7 @@ class SolverT : public Solver {
         tile_bytes = (int64_t)(bw + bh + bf + bm);
         HIPST(hipMemcpyAsync(fword, words.data(), bw, hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(halo, hl.data(), bh, hipMemcpyHostToDevice, stream));
-        if constexpr (sizeof(R) == 4)
-            HIPST(launch_mfma_pack((const float*)geo_e, (const float*)geo_f, fword, K, Kp, face4, metric4, stream));
+        HIPST(launch_mfma_pack<R>(geo_e, geo_f, fword, K, Kp, face4, metric4, stream));
         HIPST(hipStreamSynchronize(stream));
         mx.fword = fword;
         mx.halo = halo;
@@ -399,9 +402,9 @@ class SolverT : public Solver {
     // Launch shape of kernel kind 1 (lane per element), 2 (persistent
     // matrix-core tiles) or 3 (one matrix-core tile per workgroup).
     Status make_shape(int kind, LaunchShape& s) {
-        const bool mf = sizeof(R) == 4 && mfma_supported(N, p);
+        const bool mf = mfma_supported(N, p);
         if ((kind == 2 || kind == 3) && !mf)
-            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs FP32 and p = 1..4");
+            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs p = 1..4");
         if (kind < 1 || kind > 3) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1, 2 or 3");
         s = LaunchShape{};
         s.kind = kind;
@@ -414,7 +417,7 @@ class SolverT : public Solver {
             if (!st) return st;
             if (kind == 2) {
                 // persistent: as many workgroups as fit on the device at once
-                const int res = mfma_resident_groups(N, p, mx.hcap);
+                const int res = mfma_resident_groups(N, p, mx.hcap, (int)sizeof(R));
                 if (res >= 8) s.grid = std::min(s.grid, res / 8 * 8);
             }
         }
@@ -428,7 +431,7 @@ class SolverT : public Solver {
     // so the choice changes speed, not results beyond FP32 rounding.
     Status autotune(int* out_kind) {
         *out_kind = 1;
-        if (!(sizeof(R) == 4 && mfma_supported(N, p))) return Status::ok();
+        if (!mfma_supported(N, p)) return Status::ok();
         int lk = 0;
         while (lk < 30 && (1 << (lk + 1)) <= K) lk++;
         const auto key = std::make_tuple(N, p, (int)sizeof(R), lk);
@@ -496,7 +499,7 @@ class SolverT : public Solver {
             // automatic choice falls back to the lane kernel
             if (!(st = make_shape(1, s))) return st;
         }
-        const StateIx nix{N, NP, Kp, (sizeof(R) == 4 && s.kind >= 2) ? 1 : 0};
+        const StateIx nix{N, NP, Kp, s.kind >= 2 ? tile_kind() : 0};
         if (nix.tiled != six.tiled) {
             // acc, ua and ub are written in full (padding included) by a
             // stage before any stage reads them; c is carried over
@@ -549,14 +552,10 @@ class SolverT : public Solver {
         a.phys = phys;
         hipError_t e;
         hipStream_t sx = on ? on : stream;
-        if constexpr (sizeof(R) == 4) {
-            if (shape.kind >= 2)
-                e = launch_stage_mfma(N, p, a, mx, dim3(shape.grid), sx);
-            else
-                e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), sx);
-        } else {
+        if (shape.kind >= 2)
+            e = launch_stage_mfma<R>(N, p, a, mx, dim3(shape.grid), sx);
+        else
             e = launch_stage_valu<R>(N, p, a, dim3(shape.grid), sx);
-        }
         return hip_status(e, "stage kernel launch");
     }
 

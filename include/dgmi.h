@@ -188,9 +188,10 @@ int dg_solver_device_bytes(const dg_solver* s, int64_t* bytes);
  * read + write bytes.  The bandwidth reference of the roofline (no
  * counterpart in the reference). */
 int dg_bandwidth_probe(int64_t bytes, int iters, double* out);
-/* Runs one 16x16x4 FP32 matrix-core product on the GPU and compares it with
- * the host product under the lane layout the kernels assume; *max_err = max
- * absolute difference (no counterpart in the reference). */
+/* Runs one 16x16x4 FP32 and one FP64 matrix-core product on the GPU and
+ * compares them with the host product under the lane layouts the kernels
+ * assume; *max_err = max absolute difference (no counterpart in the
+ * reference). */
 int dg_selftest_mfma(double* max_err);
 
 /* ---------------- many solves, one persistent launch per RK stage ---------------- */

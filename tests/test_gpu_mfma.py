@@ -1,6 +1,6 @@
-"""Matrix-core (FP32, p = 3, 4) stage kernels against the FP64 oracle and the
+"""Matrix-core stage kernels (FP32 and FP64) against the FP64 oracle and the
 VALU kernel.  FP32 tolerance: 2e-5 relative max-norm for one RHS evaluation,
-1e-5 relative L2 after a run."""
+1e-5 relative L2 after a run; FP64: 1e-12 for both."""
 import numpy as np
 import pytest
 
@@ -15,53 +15,57 @@ def test_mfma_lane_layout():
     assert dg.selftest_mfma() == 0.0
 
 
+@pytest.mark.parametrize("prec", [dg.DG_FP32, dg.DG_FP64])
 @pytest.mark.parametrize("kind", [2, 3])
 @pytest.mark.parametrize("p", [3, 4])
 @pytest.mark.parametrize("N", [1, 4])
-def test_mfma_rhs_parity(N, p, kind):
-    print("mfma rhs N=%d p=%d kind=%d" % (N, p, kind), flush=True)
+def test_mfma_rhs_parity(N, p, kind, prec):
+    print("mfma rhs N=%d p=%d kind=%d prec=%d" % (N, p, kind, prec), flush=True)
     for m in (mesh(12), shuffled_mesh(12)):
         prob = problem_for(N)
         o = orc.Oracle.from_mesh(m, p, prob)
         u = perturbed_state(o, prob, m.period, seed=p)
         ref = o.rhs(u, dt=0.01)
         assert np.isfinite(ref).all(), "oracle reference not finite: test data out of the physical range"
-        s = dg.Solver(m, prob, p, dg.DG_FP32)
+        s = dg.Solver(m, prob, p, prec)
         s.set_kernel(kind)
         assert s.kernel() == kind
         err = rel_max(s.rhs(u, dt=0.01), ref)
         print("  rel max err %.3e" % err)
-        assert err <= 2e-5
+        assert err <= (2e-5 if prec == dg.DG_FP32 else 1e-12)
 
 
+@pytest.mark.parametrize("prec", [dg.DG_FP32, dg.DG_FP64])
 @pytest.mark.parametrize("kind", [2, 3])
-def test_mfma_run_vs_oracle(kind):
+def test_mfma_run_vs_oracle(kind, prec):
     m = shuffled_mesh(24, seed=3)
     prob = dg.euler_vortex(((3.0, 4.0, 6.0), (2.5, 7.0, 3.0)))
     o = orc.Oracle.from_mesh(m, 4, prob)
     u0 = o.project_ic(orc.ic_array(prob, m.period))
     ref, n_ref, _ = o.integrate(u0, 0.25, 0.3)
-    s = dg.Solver(m, prob, 4, dg.DG_FP32)
+    s = dg.Solver(m, prob, 4, prec)
     s.set_kernel(kind)
     s.project_ic()
     n, t = s.integrate(0.25, 0.3)
     err = orc.rel_l2(s.get_state(), ref)
-    print("kind %d steps %d/%d rel L2 %.3e" % (kind, n, n_ref, err))
-    assert t == 0.25 and err <= 1e-5
+    print("kind %d prec %d steps %d/%d rel L2 %.3e" % (kind, prec, n, n_ref, err))
+    assert t == 0.25 and err <= (1e-5 if prec == dg.DG_FP32 else 1e-12)
+    if prec == dg.DG_FP64:
+        assert n == n_ref
     a0 = o.mass(u0)
     a1 = s.audit()["mass"]
     assert np.all(np.abs(np.array(a1) - a0) <= 2e-6 * np.abs(a0) + 1e-9)
 
 
-def test_autotuner_picks_matrix_cores_for_fp32_p4():
+def test_autotuner_picks_matrix_cores_for_p4():
     """Automatic choice times the candidate kernels; at 32,768 triangles and
-    p = 4 the matrix-core kernel wins by a wide margin (the lane kernel
-    spills at this order).  FP64 has the lane kernel only."""
+    p = 4 the matrix-core kernel wins by a wide margin in both precisions
+    (the lane kernel spills at this order)."""
     m = mesh(128)
     s = dg.Solver(m, problem_for(4), 4, dg.DG_FP32)
     assert s.kernel() in (2, 3)
-    s64 = dg.Solver(mesh(8), problem_for(4), 4, dg.DG_FP64)
-    assert s64.kernel() == 1
+    s64 = dg.Solver(m, problem_for(4), 4, dg.DG_FP64)
+    assert s64.kernel() in (2, 3)
 
 
 def test_autotuned_solver_runs_correctly():
@@ -80,20 +84,21 @@ def test_autotuned_solver_runs_correctly():
         assert err <= 1e-5
 
 
+@pytest.mark.parametrize("prec", [dg.DG_FP32, dg.DG_FP64])
 @pytest.mark.parametrize("p", [1, 3, 4])
-def test_kernel_switch_carries_state(p):
+def test_kernel_switch_carries_state(p, prec):
     """The matrix-core kernels keep the FP32 state in their own tile layout
     (modes padded to a multiple of 4; p = 1, 3 pad, p = 4 pads one mode).
     Switching the kernel of a solver that holds a state converts it exactly,
     and steps taken after a switch equal steps on that kernel from the start
     (bitwise).  K = 162: the last tile is partial."""
-    print("kernel switch p=%d" % p, flush=True)
+    print("kernel switch p=%d prec=%d" % (p, prec), flush=True)
     m = shuffled_mesh(9, seed=5)
     prob = problem_for(4)
     o = orc.Oracle.from_mesh(m, p, prob)
     u = perturbed_state(o, prob, m.period, seed=11)
-    u32 = u.astype(np.float32).astype(np.float64)
-    s = dg.Solver(m, prob, p, dg.DG_FP32)
+    u32 = u.astype(np.float32).astype(np.float64) if prec == dg.DG_FP32 else u
+    s = dg.Solver(m, prob, p, prec)
     s.set_kernel(1)
     s.set_state(u)
     for kind in (2, 1, 3, 2, 1):
@@ -102,7 +107,7 @@ def test_kernel_switch_carries_state(p):
         assert np.array_equal(s.get_state(), u32), "state changed by the switch to kernel %d" % kind
     s.set_kernel(2)
     s.run_steps(3, 0.3)
-    b = dg.Solver(m, prob, p, dg.DG_FP32)
+    b = dg.Solver(m, prob, p, prec)
     b.set_kernel(2)
     b.set_state(u)
     b.run_steps(3, 0.3)
@@ -118,8 +123,8 @@ def test_tiny_meshes_every_kernel(n):
     """Meshes of 2 to 50 triangles: one partial tile, neighbours reached
     across several periodic sides, halo lists of tile elements only.  Every
     kernel, at every order the matrix-core kernels take (p = 1, 2 use a
-    single mode group of the tile layout), matches the oracle's RHS: FP64
-    lane kernel to 1e-12, FP32 kernels to 2e-5 (relative max-norm)."""
+    single mode group of the FP32 tile layout), matches the oracle's RHS:
+    FP64 kernels to 1e-12, FP32 kernels to 2e-5 (relative max-norm)."""
     m = dg.Mesh.periodic_jittered(n, n, 0.0, 10.0, 0.0, 10.0, 0.1, 7)
     for N in (1, 4):
         prob = problem_for(N)
@@ -129,8 +134,10 @@ def test_tiny_meshes_every_kernel(n):
             u = perturbed_state(o, prob, m.period, seed=n + p)
             ref = o.rhs(u, dt=0.01)
             assert np.isfinite(ref).all(), "oracle reference not finite: test data out of the physical range"
-            s64 = dg.Solver(m, prob, p, dg.DG_FP64)
-            assert rel_max(s64.rhs(u, dt=0.01), ref) <= 1e-12
+            for kind in (1, 2, 3):
+                s64 = dg.Solver(m, prob, p, dg.DG_FP64)
+                s64.set_kernel(kind)
+                assert rel_max(s64.rhs(u, dt=0.01), ref) <= 1e-12, kind
             for kind in (1, 2, 3):
                 s = dg.Solver(m, prob, p, dg.DG_FP32)
                 s.set_kernel(kind)
