@@ -72,29 +72,30 @@ __device__ __forceinline__ int run_tile(const int2* __restrict__ items, int x, i
 // the solve run alone bit for bit): the state and geometry of the next tile
 // and the halo ids of the one after are in flight while a tile computes.
 // Returns the first item index not processed.
-template <int N, int P, int KIND>
-__device__ __forceinline__ int batch_mfma_run(const StageArgs<float>& a, const MfmaArgs& mx, int slot,
-                                              const int2* __restrict__ items, int it, int i1, int G8, floatx4* fd,
-                                              float* ts, floatx4* tabA, double& lam) {
+template <int N, int P, int KIND, typename R>
+__device__ __forceinline__ int batch_mfma_run(const StageArgs<R>& a, const MfmaArgs& mx, int slot,
+                                              const int2* __restrict__ items, int it, int i1, int G8,
+                                              typename MfmaT<R>::V* fd, R* ts, typename MfmaT<R>::V* tabA,
+                                              double& lam) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
-    const float dt = (float)uni(a.ctrl->dt);
+    const R dt = (R)uni(a.ctrl->dt);
     int tile = uni(items[it].y);
     int n1 = run_tile(items, it + G8, i1, slot, tile);
-    MfmaTileState<N, P, float> st;
-    int hid[MfmaTileState<N, P, float>::HPW];
-    TileGeo<float> gc;
-    MfmaTileState<N, P, float>::load_ids(mx.halo, mx.hcap, tile, w, j, hid);
+    MfmaTileState<N, P, R> st;
+    int hid[MfmaTileState<N, P, R>::HPW];
+    TileGeo<R> gc;
+    MfmaTileState<N, P, R>::load_ids(mx.halo, mx.hcap, tile, w, j, hid);
     gc.load(a, mx, tile, w * 16 + j);
     st.load_data(a.u, a.K, tile, w, g, j, hid);
-    MfmaTileState<N, P, float>::load_ids(mx.halo, mx.hcap, n1, w, j, hid);
-    if (w == 0) mfma_a_to_lds<N, P, float>(tabA, lane);  // the previous run ended with a barrier
+    MfmaTileState<N, P, R>::load_ids(mx.halo, mx.hcap, n1, w, j, hid);
+    if (w == 0) mfma_a_to_lds<N, P, R>(tabA, lane);  // the previous run ended with a barrier
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     while (true) {
         const int nit = it + G8;
         const bool more = nit < i1 && uni(items[nit].x) == slot;
         const int n2 = run_tile(items, it + 2 * G8, i1, slot, n1);
-        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND, float>(a, mx, tabA, st, hid, gc, dt, tile, n1, n2, fd, ts));
+        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND, R>(a, mx, tabA, st, hid, gc, dt, tile, n1, n2, fd, ts));
         if (!more) return nit;
         it = nit;
         tile = n1;
@@ -168,12 +169,6 @@ __device__ __forceinline__ void batch_lane_items(const SlotDesc<R>* __restrict__
         const int N = uni(sd->N), p = uni(sd->p);
         const int e = uni(item.y) * BATCH_VALU_TILE + threadIdx.x;
         double lam = 0.0;
-        if constexpr (sizeof(R) == 8) {
-            if (N == 4 && p == 4) lam = valu_process<4, 4, R>(a, e, KIND);
-            else if (N == 4 && p == 3) lam = valu_process<4, 3, R>(a, e, KIND);
-            else if (N == 1 && p == 4) lam = valu_process<1, 4, R>(a, e, KIND);
-            else if (N == 1 && p == 3) lam = valu_process<1, 3, R>(a, e, KIND);
-        }
         if (N == 4 && p == 2) lam = valu_process<4, 2, R>(a, e, KIND);
         else if (N == 4 && p == 1) lam = valu_process<4, 1, R>(a, e, KIND);
         else if (N == 1 && p == 2) lam = valu_process<1, 2, R>(a, e, KIND);
@@ -186,35 +181,22 @@ __device__ __forceinline__ void batch_lane_items(const SlotDesc<R>* __restrict__
     }
 }
 
-// One persistent launch per stage.  Each workgroup first takes its share of
-// its XCD's lane items, then of its tile items; the two walks are separate
-// loops so the register allocation of one body does not burden the other.
-// FP32 solves use lane items for p <= 2 and tiles for p >= 3 (the kernel
-// choice of a single solve, so results match it bit for bit); FP64 solves
-// use lane items at every order.
-template <int KIND>
-__global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float>* __restrict__ slots,
-                                                           const int2* __restrict__ items,
-                                                           const int* __restrict__ parts) {
+// Tile items of XCD x: runs of consecutive tiles of one solve through the
+// pipelined matrix-core body.
+template <typename R, int KIND>
+__device__ __forceinline__ void batch_tile_items(const SlotDesc<R>* __restrict__ slots,
+                                                 const int2* __restrict__ items, int it, int i1, int G8,
+                                                 typename MfmaT<R>::V* fd, R* ts, typename MfmaT<R>::V* tabA,
+                                                 double* sh) {
     constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
-    extern __shared__ float4 lds4[];
-    __shared__ double sh[8];
-    floatx4* fd = reinterpret_cast<floatx4*>(lds4);
-    float* ts = reinterpret_cast<float*>(fd + MFMA_TILE * 3);
-    __shared__ floatx4 tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
-    const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
-    batch_lane_items<float, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
-
-    const int i1 = parts[BATCH_TILE_PART + x + 1];
-    int it = parts[BATCH_TILE_PART + x] + bi;
     while (it < i1) {
         const int slot = uni(items[it].x);
-        const SlotDesc<float>* sd = slots + slot;
+        const SlotDesc<R>* sd = slots + slot;
         if (!uni(uni(sd->ctrl)->active)) {
             it += G8;
             continue;
         }
-        const StageArgs<float> a = slot_args<float, KIND>(sd);
+        const StageArgs<R> a = slot_args<R, KIND>(sd);
         MfmaArgs mx;
         mx.fword = uni(sd->fword);
         mx.halo = uni(sd->halo);
@@ -225,10 +207,10 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float
         const int N = uni(sd->N), p = uni(sd->p);
         double lam = 0.0;
         int next = it + G8;
-        if (N == 4 && p == 4) next = batch_mfma_run<4, 4, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
-        else if (N == 4 && p == 3) next = batch_mfma_run<4, 3, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
-        else if (N == 1 && p == 4) next = batch_mfma_run<1, 4, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
-        else if (N == 1 && p == 3) next = batch_mfma_run<1, 3, KIND>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        if (N == 4 && p == 4) next = batch_mfma_run<4, 4, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        else if (N == 4 && p == 3) next = batch_mfma_run<4, 3, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        else if (N == 1 && p == 4) next = batch_mfma_run<1, 4, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        else if (N == 1 && p == 3) next = batch_mfma_run<1, 3, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
         it = next;
         if constexpr (LAST) {
             const double m = block_max(lam, sh);
@@ -238,13 +220,39 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float
     }
 }
 
+// One persistent launch per stage.  Each workgroup first takes its share of
+// its XCD's lane items, then of its tile items; the two walks are separate
+// loops so the register allocation of one body does not burden the other.
+// Solves use lane items for p <= 2 and tiles for p >= 3 (the kernel choice
+// of a single solve, so results match it bit for bit).
+template <int KIND>
+__global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float>* __restrict__ slots,
+                                                           const int2* __restrict__ items,
+                                                           const int* __restrict__ parts) {
+    extern __shared__ float4 lds4[];
+    __shared__ double sh[8];
+    floatx4* fd = reinterpret_cast<floatx4*>(lds4);
+    float* ts = reinterpret_cast<float*>(fd + MFMA_TILE * 3);
+    __shared__ floatx4 tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
+    const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
+    batch_lane_items<float, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
+    batch_tile_items<float, KIND>(slots, items, parts[BATCH_TILE_PART + x] + bi, parts[BATCH_TILE_PART + x + 1], G8,
+                                  fd, ts, tabA, sh);
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(256, 1) k_batch_stage_f64(const SlotDesc<double>* __restrict__ slots,
                                                            const int2* __restrict__ items,
                                                            const int* __restrict__ parts) {
+    extern __shared__ float4 lds4[];
     __shared__ double sh[8];
+    doublex4* fd = reinterpret_cast<doublex4*>(lds4);
+    double* ts = reinterpret_cast<double*>(fd + MFMA_TILE * 3);
+    __shared__ doublex4 tabA[MFMA_A_SLOTS * 64];
     const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
     batch_lane_items<double, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
+    batch_tile_items<double, KIND>(slots, items, parts[BATCH_TILE_PART + x] + bi, parts[BATCH_TILE_PART + x + 1],
+                                   G8, fd, ts, tabA, sh);
 }
 
 template <typename R, int KIND>
@@ -324,7 +332,15 @@ class EngineT : public BatchEngine {
         HIPST(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPST(hipMalloc((void**)&d_xcd, BATCH_PARTS * sizeof(int)));
         HIPST(hipMemsetAsync(d_xcd, 0, BATCH_PARTS * sizeof(int), st));
-        lds = (sizeof(R) == 4) ? mfma_tile_lds_bytes(4, MFMA_MAX_HALO, 4) : 0;
+        lds = mfma_tile_lds_bytes(4, MFMA_MAX_HALO, (int)sizeof(R));
+        if constexpr (sizeof(R) == 8) {  // above 64 KB: raise the kernels' dynamic LDS limit
+            int mxl = 0, dv = 0;
+            HIPST(hipGetDevice(&dv));
+            HIPST(hipDeviceGetAttribute(&mxl, hipDeviceAttributeMaxSharedMemoryPerBlock, dv));
+            for (const void* f : {BatchKernel<R, ST_RK4_1>::fn(), BatchKernel<R, ST_RK4_2>::fn(),
+                                  BatchKernel<R, ST_RK4_3>::fn(), BatchKernel<R, ST_RK4_4>::fn()})
+                HIPST(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, mxl - 16 * 1024));
+        }
         int per_cu = 0, dev = 0, cus = 0;
         HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BatchKernel<R, ST_RK4_2>::fn(), 256, lds));
         HIPST(hipGetDevice(&dev));
@@ -377,9 +393,9 @@ class EngineT : public BatchEngine {
         for (int s = 0; s < ns; s++) {
             SlotInfo si;
             solvers[s]->slot_info(&si);
-            const bool tiles = sizeof(R) == 4 && si.kind >= 2;
-            if (sizeof(R) == 4 && (tiles != (si.p >= 3)))
-                return Status::err(DG_ERR_STATE, "FP32 batch slots use matrix-core tiles exactly when p >= 3");
+            const bool tiles = si.kind >= 2;
+            if (tiles != (si.p >= 3))
+                return Status::err(DG_ERR_STATE, "batch slots use matrix-core tiles exactly when p >= 3");
             if (tiles && si.hcap > MFMA_MAX_HALO) return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
             SlotDesc<R>& d = hs[s];
             d.c = (R*)si.c;
@@ -544,7 +560,7 @@ Status make_batch_solver(const Mesh& m, const dg_problem_desc& prob, int p, int 
     if (p < 1 || p > 4) return Status::err(DG_ERR_UNSUPPORTED, "batched solves support p = 1..4");
     // the batch's fixed per-order choice (no autotuning launches at setup)
     Solver* s = nullptr;
-    Status st = Solver::create(m, prob, p, precision, &s, (precision == DG_FP32 && p >= 3) ? 2 : 1);
+    Status st = Solver::create(m, prob, p, precision, &s, p >= 3 ? 2 : 1);
     if (!st) return st;
     *out = s;
     return Status::ok();
