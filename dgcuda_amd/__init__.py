@@ -466,7 +466,7 @@ def request(nx, p, t_end, prob, domain=(0.0, 10.0), jitter=0.1, mesh_seed=0, arr
 
 
 def selftest_mfma():
-    """Max abs error of a 16x16x4 FP32 matrix-core product vs the host."""
+    """Max abs error of a 16x16x4 FP32 and an FP64 matrix-core product vs the host."""
     e = C.c_double()
     _check(lib().dg_selftest_mfma(C.byref(e)))
     return e.value

@@ -6,9 +6,10 @@
 // Every solve keeps its own buffers, geometry and control block (its own
 // CFL dt, time and end time), so a solve's arithmetic does not depend on
 // which other solves share the launch: its result is the same alone and in
-// any mix.  In FP32 every item runs the matrix-core tile body (64-element
-// tiles; the products are padded to 16x16 at low order), in FP64 the
-// one-element-per-lane body (256-element tiles).  The work list is split into 8
+// any mix.  Solves of p >= 3 run as 64-element tiles through the pipelined
+// matrix-core body of the single-solve kernel (FP32 or FP64), solves of
+// p <= 2 as 256-element lane items of the one-element-per-lane body, the
+// single solve's kernel choice in both cases.  The work list is split into 8
 // contiguous chunks of equal estimated cost, one per XCD, and the
 // workgroups of an XCD take the items of its chunk round-robin.
 #include <algorithm>

@@ -226,7 +226,8 @@ class SolverT : public Solver {
     int64_t bytes = 0;
     int audit_blocks = 0;
     // layout of c, acc, ua, ub: mode-major for the lane kernel, the tile
-    // layout for the matrix-core kernels (FP32); switched with the kernel
+    // layout of the precision for the matrix-core kernels; switched with the
+    // kernel
     StateIx six{};
     bool state_set = false;  // c holds a state (projected, set or stepped)
     double b0 = 0;  // phi_0 at the first volume point (cell mean = c_0 * b0)
@@ -427,8 +428,8 @@ class SolverT : public Solver {
     // Automatic kernel choice: time every kind this solve supports on its
     // own buffers (3 right-hand-side launches after a warm-up, HIP events)
     // and take the fastest; cached per (physics, p, precision, log2 K).
-    // FP64 has one kernel.  The candidates agree to the stated tolerances,
-    // so the choice changes speed, not results beyond FP32 rounding.
+    // The candidates agree to the stated tolerances (FP32 rounding, or 1e-12
+    // in FP64), so the choice changes speed, not results beyond those.
     Status autotune(int* out_kind) {
         *out_kind = 1;
         if (!mfma_supported(N, p)) return Status::ok();

@@ -164,9 +164,10 @@ int dg_audit(dg_solver* s, dg_audit_out* out);
 /* uv[3*num_elem]: variable var at the three vertices of each element (eval_u) */
 int dg_eval_vertices(dg_solver* s, int var, double* uv);
 int dg_solver_sync(dg_solver* s);
-/* kernel selection for the stage kernels: 0 = automatic (tuned table),
- * 1 = one element per lane (VALU), 2 / 3 = matrix-core tiles of 64 / 128
- * elements (FP32, p = 3, 4) */
+/* kernel selection for the stage kernels: 0 = automatic (timed per
+ * physics, order, precision and size), 1 = one element per lane (VALU),
+ * 2 = persistent matrix-core tiles of 64 elements, 3 = one matrix-core tile
+ * per workgroup (2 and 3: FP32 or FP64, p = 1..4) */
 int dg_solver_set_kernel(dg_solver* s, int kind);
 int dg_solver_get_kernel(const dg_solver* s, int* kind);
 /* Times nsteps RK4 steps with HIP events on the solver's stream, one event
