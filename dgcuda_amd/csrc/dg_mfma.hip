@@ -64,12 +64,12 @@ hipError_t mfma_layout_probe_f64(double* host_out) {
 // occupancy target (waves per SIMD) the register allocation is held to
 // at least 3 workgroups (12 wavefronts) per CU in FP32: the register budget
 // the p = 4 body fits without spills, and the faster setting on MI355X.
-// FP64 doubles every value's registers: 1 workgroup per CU.
+// FP64 doubles every value's registers: 2 workgroups per CU (256 registers).
 #ifndef DGMI_MFMA_MINW
 #define DGMI_MFMA_MINW 3
 #endif
 template <typename R>
-constexpr int mfma_minw() { return sizeof(R) == 4 ? DGMI_MFMA_MINW : 1; }
+constexpr int mfma_minw() { return sizeof(R) == 4 ? DGMI_MFMA_MINW : 2; }
 
 template <int N, int P, int KIND, typename R>
 __global__ void __launch_bounds__(256, mfma_minw<R>()) k_stage_mfma(StageArgs<R> a, MfmaArgs mx) {

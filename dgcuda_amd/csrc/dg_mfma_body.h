@@ -384,8 +384,10 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
         cr[v] = V{0, 0, 0, 0};
         ar[v] = V{0, 0, 0, 0};
         if constexpr (RK::C_FROM_INPUT) cr[v] = st.ub[v];
-        if constexpr (RK::LOAD_C) cr[v] = __builtin_nontemporal_load(&gp4(a.c)[v * VS + l4]);
-        if constexpr (RK::LOAD_ACC) ar[v] = __builtin_nontemporal_load(&gp4(a.acc)[v * VS + l4]);
+        if constexpr (sizeof(R) == 4) {
+            if constexpr (RK::LOAD_C) cr[v] = __builtin_nontemporal_load(&gp4(a.c)[v * VS + l4]);
+            if constexpr (RK::LOAD_ACC) ar[v] = __builtin_nontemporal_load(&gp4(a.acc)[v * VS + l4]);
+        }
     }
     __builtin_amdgcn_sched_barrier(0);
     st.load_data(a.u, K, next, w, g, j, hid);
@@ -454,6 +456,17 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
     const R cj = inv_jacobian(gc.J) * dt;
     __builtin_amdgcn_s_setprio(2);
     __builtin_amdgcn_sched_barrier(0);
+    if constexpr (sizeof(R) == 8) {
+        // FP64 requests c and acc here instead, after the fluxes: their 64
+        // registers are then not live across the face fluxes, which lets two
+        // workgroups share a CU (0.409 -> 0.323 ms per stage at 524,288
+        // triangles, p = 4); FP32 measured no gain from the same move
+#pragma unroll
+        for (int v = 0; v < N; v++) {
+            if constexpr (RK::LOAD_C) cr[v] = __builtin_nontemporal_load(&gp4(a.c)[v * VS + l4]);
+            if constexpr (RK::LOAD_ACC) ar[v] = __builtin_nontemporal_load(&gp4(a.acc)[v * VS + l4]);
+        }
+    }
     gc.load(a, mx, next, el);
     MfmaTileState<N, P>::load_ids(mx.halo, mx.hcap, next2, w, j, hid);
     __builtin_amdgcn_sched_barrier(0);
