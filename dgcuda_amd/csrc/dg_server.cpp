@@ -33,11 +33,13 @@ namespace dgmi {
 static int64_t estimate_bytes(int K, int N, int p, int precision) {
     const int64_t Kp = (K + 63) / 64 * 64;
     const int64_t s = precision == DG_FP64 ? 8 : 4;
-    // FP32 at p <= 4 reserves the matrix-core tile layout (modes padded to a multiple of 4)
-    const int64_t np = (precision == DG_FP32 && p >= 1 && p <= 4) ? 4 * tile_groups(n_modes(p)) : n_modes(p);
+    // p <= 4 reserves the matrix-core tile layout: modes padded to a multiple
+    // of 4 in FP32, to 16 in FP64
+    const bool tl = p >= 1 && p <= 4;
+    const int64_t np = !tl ? n_modes(p) : precision == DG_FP32 ? 4 * tile_groups(n_modes(p)) : 16;
     const int64_t ntiles = (K + 63) / 64;
-    // matrix-core tiles (FP32, p >= 3 in a batch): packed geometry, 64 bytes per element
-    const int64_t packed = (precision == DG_FP32 && p >= 3) ? 64 : 0;
+    // matrix-core tiles (p >= 3 in a batch): packed geometry, 16 values per element
+    const int64_t packed = p >= 3 ? 16 * s : 0;
     return Kp * (4 * N * np * s + 14 * s + 3 * 4 + 6 * 8 + 8 + 8 + 3 * 2 + packed) + ntiles * 64 * 4 + 4096;
 }
 
