@@ -428,13 +428,22 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
         for (int v = 0; v < N; v++) Uq[v] = U[v][i];
         const bool qv = M::pos(g, i) < NQ;
         if (N == 4 && !qv) Uq[0] = R(1);  // padding point: keep the flux finite
-        flux<N, R>(a.phys, Uq, fx, fy);
+        if constexpr (sizeof(R) == 4 && N == 4) {
+            metric_flux(a.phys, Uq, xr, yr, xs, ys, fx, fy);  // fx, fy hold Fr, Fs
 #pragma unroll
-        for (int v = 0; v < N; v++) {
-            const R fr = qv ? fx[v] * ys - fy[v] * xs : R(0);
-            const R fs = qv ? fy[v] * xr - fx[v] * yr : R(0);
-            Rv[v] = M::mma(Agx[i], fr, Rv[v]);
-            Rv[v] = M::mma(Agy[i], fs, Rv[v]);
+            for (int v = 0; v < N; v++) {
+                Rv[v] = M::mma(Agx[i], qv ? fx[v] : R(0), Rv[v]);
+                Rv[v] = M::mma(Agy[i], qv ? fy[v] : R(0), Rv[v]);
+            }
+        } else {
+            flux<N, R>(a.phys, Uq, fx, fy);
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                const R fr = qv ? fx[v] * ys - fy[v] * xs : R(0);
+                const R fs = qv ? fy[v] * xr - fx[v] * yr : R(0);
+                Rv[v] = M::mma(Agx[i], fr, Rv[v]);
+                Rv[v] = M::mma(Agy[i], fs, Rv[v]);
+            }
         }
     }
 #pragma unroll
@@ -450,21 +459,43 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
             const bool left = (wd >> 14) & 1;
             const R* own = ts + el * TS + pt * N;
             const R* nbt = ts + slot * TS + (fn * NQ1 + NQ1 - 1 - jo) * N;
-            R UL[N], UR[N];
+            if constexpr (sizeof(R) == 4 && N == 4) {
+                // F(UL).n + F(UR).n, max of the two speeds and UL - UR with the
+                // sign of this side folded into the speed: sums and maxima of
+                // the two sides' terms commute and the sign flips are exact, so
+                // both elements of the side still get bit-identical values with
+                // opposite signs
+                R uo[N], un[N], Fo[N], Fb[N], so, sb;
 #pragma unroll
-            for (int v = 0; v < N; v++) {
-                const R uo = own[v], un = nbt[v];
-                UL[v] = left ? uo : un;
-                UR[v] = left ? un : uo;
+                for (int v = 0; v < N; v++) {
+                    uo[v] = own[v];
+                    un[v] = nbt[v];
+                }
+                normal_flux_speed(a.phys, uo, r[0], r[1], Fo, so);
+                normal_flux_speed(a.phys, un, r[0], r[1], Fb, sb);
+                const R lm = fmaxf(so, sb);
+                const R lms = left ? lm : -lm;
+                const R cf = left ? R(-0.25) * r[2] : R(0.25) * r[2];
+#pragma unroll
+                for (int v = 0; v < N; v++) G[v] = cf * fma(lms, uo[v] - un[v], Fo[v] + Fb[v]);
+            } else {
+                R UL[N], UR[N];
+#pragma unroll
+                for (int v = 0; v < N; v++) {
+                    const R uo = own[v], un = nbt[v];
+                    UL[v] = left ? uo : un;
+                    UR[v] = left ? un : uo;
+                }
+                R fxl[N], fyl[N], fxr[N], fyr[N];
+                flux<N, R>(a.phys, UL, fxl, fyl);
+                flux<N, R>(a.phys, UR, fxr, fyr);
+                const R lm = llf_speed<N, R>(a.phys, UL, UR, r[0], r[1]);
+                const R coef = left ? R(-0.5) * r[2] : R(0.5) * r[2];
+#pragma unroll
+                for (int v = 0; v < N; v++)
+                    G[v] = coef *
+                           (R(0.5) * ((fxl[v] + fxr[v]) * r[0] + (fyl[v] + fyr[v]) * r[1] + lm * (UL[v] - UR[v])));
             }
-            R fxl[N], fyl[N], fxr[N], fyr[N];
-            flux<N, R>(a.phys, UL, fxl, fyl);
-            flux<N, R>(a.phys, UR, fxr, fyr);
-            const R lm = llf_speed<N, R>(a.phys, UL, UR, r[0], r[1]);
-            const R coef = left ? R(-0.5) * r[2] : R(0.5) * r[2];
-#pragma unroll
-            for (int v = 0; v < N; v++)
-                G[v] = coef * (R(0.5) * ((fxl[v] + fxr[v]) * r[0] + (fyl[v] + fyr[v]) * r[1] + lm * (UL[v] - UR[v])));
         } else {
 #pragma unroll
             for (int v = 0; v < N; v++) G[v] = R(0);
