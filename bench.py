@@ -33,9 +33,22 @@ def vortices(seed, n=3, L=10.0):
     return tuple((float(rng.uniform(2, 5)), float(rng.uniform(0, L)), float(rng.uniform(0, L))) for _ in range(n))
 
 
-def algorithmic_bytes_per_elem(N, n_p, s, kernel):
-    """Bytes one RK4 stage must move per element, averaged over the 4 stages
-    (each input read once, each output written once; halo re-reads are not
+def algorithmic_bytes_per_elem(N, n_p, s):
+    """Algorithmic bytes of one RK4 stage per element, the reference
+    algorithm's byte model: state and RK registers 4.25*N*n_p*s (classic RK4
+    with a fused stage update: each stage reads its input once; stage 1
+    writes acc and u2; stages 2-3 read u_n and acc and write acc and the
+    next input; stage 4 reads u_n and acc and writes u_n+1, i.e. (1+2, 1+4,
+    1+4, 1+3)/4 = 4.25 streams) plus 110 B of geometry (FP64 metrics: 5x8 B
+    per element, 3x4 B element-to-face indices, 1.5 faces x (3x8 B normal
+    and length + 3x4 B connectivity), CFL cell means N*s/4).  Face-flux
+    buffers are not counted.  C3 FP32: 18.83 B per DOF*stage."""
+    return 4.25 * N * n_p * s + 110.0
+
+
+def kernel_bytes_per_elem(N, n_p, s, kernel):
+    """Bytes the kernels of this build must move per element and stage
+    (each input read once, each output written once; halo re-reads not
     counted).  Matrix-core path (kernel 2/3), RK4 in stage states: state
     passes 3.0*N*n_p*s (stage 1 reads u_n and writes u2; stage 2 reads u2
     and u_n and writes u3 and W = (2 u_n + (u2 - u_n) + dt k2)/3; stage 3
@@ -192,15 +205,19 @@ def main():
     # kernels and read longer.
     tk = s.time_stage_kernels(20, 0.3)
     stage_ms = (tg["step_ms"] - tk["dt_ms"]) / 4
-    bpe = algorithmic_bytes_per_elem(N, n_p, s_bytes, s.kernel())
+    bpe = algorithmic_bytes_per_elem(N, n_p, s_bytes)
     bytes_launch = bpe * K
     achieved = bytes_launch / (stage_ms * 1e-3) / 1e9
+    kbytes_launch = kernel_bytes_per_elem(N, n_p, s_bytes, s.kernel()) * K
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                 frac=round(achieved / PEAK_HBM_GBS, 4), traffic=load_traffic(args, s.kernel()),
                 kernel="stage (fused volume + LLF faces + M^-1 + RK update)",
                 stage_ms=round(stage_ms, 5), stage_ms_direct=round(tk["stage_ms"], 5),
                 graph_step_ms=round(tg["step_ms"], 5), dt_kernel_ms=round(tk["dt_ms"], 5),
-                algorithmic_bytes_per_launch=int(bytes_launch), bytes_per_dof_stage=round(bpe / (n_p * N), 3))
+                algorithmic_bytes_per_launch=int(bytes_launch), bytes_per_dof_stage=round(bpe / (n_p * N), 3),
+                kernel_bytes_per_launch=int(kbytes_launch),
+                kernel_bytes_GBs=round(kbytes_launch / (stage_ms * 1e-3) / 1e9, 1),
+                kernel_bytes_frac=round(kbytes_launch / (stage_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4))
 
     extra = {}
     if state32 is not None and prec == dg.DG_FP32:
@@ -241,7 +258,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(bw, 1),
-        "frac_of_copy": round(achieved / bw, 4),
+        # measured HBM traffic per launch over the live duration, against the copy probe
+        "traffic_GBs": None if roof["traffic"] is None else round(roof["traffic"] / (stage_ms * 1e-3) / 1e9, 1),
+        "traffic_frac_of_copy": None if roof["traffic"] is None else round(roof["traffic"] / (stage_ms * 1e-3) / 1e9 / bw, 4),
         "setup_s": round(setup_s, 3),
         "t_reached": t_reached,
         "host": platform.node(),
