@@ -27,8 +27,8 @@ DG_OK = 0
 DG_PHYS_ADVECTION, DG_PHYS_EULER = 1, 4
 DG_FP64, DG_FP32 = 0, 1
 DG_RK4, DG_RK2 = 1, 2
-DG_IC_CONSTANT, DG_IC_GAUSSIAN, DG_IC_VORTEX = 0, 1, 2
-DG_QUIRK_FLOAT_SQRT, DG_QUIRK_FLOAT_GEOM = 1, 2
+DG_IC_CONSTANT, DG_IC_GAUSSIAN, DG_IC_VORTEX, DG_IC_SUPERSONIC = 0, 1, 2, 3
+DG_QUIRK_FLOAT_SQRT, DG_QUIRK_FLOAT_GEOM, DG_QUIRK_FLOAT_POW, DG_QUIRK_REFLECT_MIRROR = 1, 2, 4, 8
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_IO", 3: "DG_ERR_PARSE", 4: "DG_ERR_HIP",
                 5: "DG_ERR_UNSUPPORTED", 6: "DG_ERR_NUMERIC", 7: "DG_ERR_BUDGET", 8: "DG_ERR_STATE"}
 STATUS_CODES = {v: k for k, v in STATUS_NAMES.items()}
@@ -47,7 +47,7 @@ class ProblemDesc(C.Structure):
         ("gauss_x", C.c_double), ("gauss_y", C.c_double), ("gauss_k", C.c_double),
         ("rho_inf", C.c_double), ("u_inf", C.c_double), ("v_inf", C.c_double), ("p_inf", C.c_double),
         ("vortex_beta", C.c_double * 4), ("vortex_x", C.c_double * 4), ("vortex_y", C.c_double * 4),
-        ("constant", C.c_double * 4),
+        ("constant", C.c_double * 4), ("mach", C.c_double),
     ]
 
 
@@ -255,6 +255,15 @@ def euler_vortex(vortices=((5.0, 5.0, 5.0),), gamma=1.4, background=(1.0, 1.0, 1
 
 def euler_constant(U, gamma=1.4, quirks=0):
     return problem(DG_PHYS_EULER, DG_IC_CONSTANT, gamma=gamma, constant=U, quirks=quirks)
+
+
+def supersonic_vortex(mach=2.25, gamma=1.4, quirks=0):
+    """Supersonic vortex between the arcs r = 1 and r = 1.384
+    (euler/supersonic/supersonic.cu): the exact steady state is the initial
+    state, which the inflow and outflow sides impose; walls reflect."""
+    d = problem(DG_PHYS_EULER, DG_IC_SUPERSONIC, gamma=gamma, quirks=quirks)
+    d.mach = mach
+    return d
 
 
 class Solver:

@@ -119,6 +119,7 @@ __device__ __forceinline__ StageArgs<R> slot_args(const SlotDesc<R>* sd) {
     a.ntiles = 0;
     a.xcd_remap = 0;
     a.pad = 0;
+    a.bnd = nullptr;  // batched solves have no boundary sides
     a.phys.gamma = uni(sd->phys.gamma);
     a.phys.ax = uni(sd->phys.ax);
     a.phys.ay = uni(sd->phys.ay);
@@ -559,6 +560,9 @@ Status BatchEngine::create(int precision, int max_slots, int max_items, BatchEng
 
 Status make_batch_solver(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out) {
     if (p < 1 || p > 4) return Status::err(DG_ERR_UNSUPPORTED, "batched solves support p = 1..4");
+    for (int s = 0; s < m.S; s++)
+        if (m.right_elem[s] < 0)
+            return Status::err(DG_ERR_UNSUPPORTED, "batched solves need meshes without boundary sides");
     // the batch's fixed per-order choice (no autotuning launches at setup)
     Solver* s = nullptr;
     Status st = Solver::create(m, prob, p, precision, &s, p >= 3 ? 2 : 1);

@@ -135,9 +135,21 @@ struct PhysP {
 
 // Packed neighbour word per (element, face): bits 0..27 neighbour element,
 // bits 28..29 neighbour's local face, bit 30 set when this element is the
-// side's left element.
+// side's left element.  A boundary side (the element is its left element)
+// has bit 31 set (a negative word), bits 28..29 the boundary kind (1
+// reflecting, 2 outflow, 3 inflow: minus the reference's right_elem code)
+// and bits 0..27 the index of the side in the boundary data.
 constexpr int NB_ELEM_MASK = (1 << 28) - 1;
 __host__ __device__ inline int nb_pack(int e, int f, int is_left) { return e | (f << 28) | (is_left << 30); }
+__host__ __device__ inline int nb_pack_boundary(int bfi, int kind) {
+    return (int)((unsigned)bfi | ((unsigned)kind << 28) | (1u << 30) | (1u << 31));
+}
+enum { BC_REFLECTING = 1, BC_OUTFLOW = 2, BC_INFLOW = 3 };
+// Highest order whose lane kernel carries the boundary path.  The first
+// run of the p = 4 FP64 instance with it faulted the GPU (memory aperture
+// violation) and the cause is not found, so p >= 4 keeps the kernel without
+// it and the solver refuses boundary meshes there.
+constexpr int VALU_BOUNDARY_MAX_P = 3;
 
 enum StageKind {
     ST_RHS = 0,   // out = dt * M^-1 R(u)
@@ -196,6 +208,10 @@ struct StageArgs {
     int xcd_remap;    // map consecutive tiles to one XCD
     int pad;
     PhysP phys;
+    // boundary sides, [side][point][4] FP64 (lane kernel only): the ghost
+    // state of inflow / outflow points, the wall normal (Nx, Ny, 0, 0) of
+    // reflecting points; null for a mesh without boundary sides
+    const double* bnd;
 };
 
 // Workgroup -> tile mapping.  The dispatcher places workgroup b on XCD b % 8;

@@ -61,10 +61,11 @@ double mesh_geometry(const Mesh& m, int quirks, std::vector<double>& geo_e, std:
 std::string elem_geometry(const Mesh& m, const std::vector<int>& perm, int quirks, ElemGeometry& g) {
     const int K = m.K;
     for (int s = 0; s < m.S; s++)
-        if (m.right_elem[s] < 0)
-            return "mesh has boundary sides (right element " + std::to_string(m.right_elem[s]) +
-                   "); only periodic / closed meshes are supported";
+        if (m.right_elem[s] < -3)
+            return "side " + std::to_string(s) + " has boundary code " + std::to_string(m.right_elem[s]) +
+                   " (reflecting -1, outflow -2 and inflow -3 are known)";
     if (K >= (1 << 28)) return "mesh too large";
+    g.bfaces.clear();
     std::vector<double> ge, gs;
     g.min_r = mesh_geometry(m, quirks, ge, gs);
     g.K = K;
@@ -93,7 +94,14 @@ std::string elem_geometry(const Mesh& m, const std::vector<int>& perm, int quirk
             g.fnx[f][n] = gs[m.S + s];
             g.fny[f][n] = gs[2 * (size_t)m.S + s];
             g.flen[f][n] = gs[s];
-            g.nbr[f][n] = (inv[other]) | (ofc << 28) | ((left ? 1 : 0) << 30);
+            if (other < 0) {  // boundary side: this element is its left element
+                // the kernels' boundary word: side index, kind in bits 28..29,
+                // left flag (bit 30), bit 31 set
+                g.nbr[f][n] = (int)((unsigned)g.bfaces.size() | ((unsigned)-other << 28) | (1u << 30) | (1u << 31));
+                g.bfaces.push_back(ElemGeometry::BFace{n, f, -other});
+            } else {
+                g.nbr[f][n] = (inv[other]) | (ofc << 28) | ((left ? 1 : 0) << 30);
+            }
         }
     }
     return std::string();

@@ -25,6 +25,13 @@ struct ElemGeometry {
     std::vector<int> nbr[3];
     std::vector<double> V[6];
     double min_r = 0;
+    // boundary sides in the order of their index in the neighbour word:
+    // element (kernel order), its local face, kind (1 reflecting, 2 outflow,
+    // 3 inflow)
+    struct BFace {
+        int elem, face, kind;
+    };
+    std::vector<BFace> bfaces;
 };
 
 std::string elem_geometry(const Mesh& m, const std::vector<int>& perm, int quirks, ElemGeometry& g);

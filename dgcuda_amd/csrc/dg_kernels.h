@@ -4,9 +4,10 @@
 
 namespace dgmi {
 
-// initial-condition parameters passed by value to the projection kernel
+// initial-condition parameters passed by value to the projection kernel;
+// also the state of inflow and outflow boundary sides
 struct IcP {
-    int kind;   // 0 constant, 1 Gaussian pulse, 2 vortex superposition
+    int kind;   // 0 constant, 1 Gaussian pulse, 2 vortex superposition, 3 supersonic vortex
     int nv;
     double gamma;
     double gx, gy, gk;
@@ -14,7 +15,65 @@ struct IcP {
     double rho_inf, u_inf, v_inf, p_inf;
     double beta[4], vx[4], vy[4];
     double constant[4];
+    double mach;  // supersonic vortex: Mach number at r = 1
+    int fpow;     // supersonic vortex: powers rounded through float
+    int pad;
 };
+
+// Initial condition U0(x, y) in FP64 (conserv_kernels.cu:167-187,
+// euler.cu:31-59, supersonic.cu:28-38), on the host and the device.
+__host__ __device__ inline double wrap_d(double d, double L) { return (L > 0) ? d - L * rint(d / L) : d; }
+
+// a^b, or with fpow the reference's powf on double arguments: arguments and
+// result rounded to float (the power itself here in FP64)
+__host__ __device__ inline double ic_pow(double a, double b, int fpow) {
+    return fpow ? (double)(float)pow((double)(float)a, (double)(float)b) : pow(a, b);
+}
+
+__host__ __device__ inline void ic_point(const IcP& ic, double x, double y, double* U) {
+    if (ic.kind == 0) {
+        for (int n = 0; n < 4; n++) U[n] = ic.constant[n];
+        return;
+    }
+    if (ic.kind == 1) {
+        double dx = wrap_d(x - ic.gx, ic.Lx), dy = wrap_d(y - ic.gy, ic.Ly);
+        U[0] = exp(-ic.gk * (dx * dx + dy * dy));
+        return;
+    }
+    if (ic.kind == 3) {
+        // supersonic vortex between two arcs (supersonic.cu:28-38): density
+        // (1 + (gamma-1)/2 M^2 (1 - 1/r^2))^(1/(gamma-1)) with the reference's
+        // literals 1.0125 and 2.5, speed M / r along the arcs, p = rho^gamma / gamma
+        const double r = sqrt(x * x + y * y);
+        U[0] = ic_pow(1 + 1.0125 * (1. - 1. / (r * r)), 2.5, ic.fpow);
+        U[1] = U[0] * sin(atan(y / x)) * ic.mach / r;
+        U[2] = U[0] * -cos(atan(y / x)) * ic.mach / r;
+        const double pr = (1.0 / ic.gamma) * ic_pow(U[0], ic.gamma, ic.fpow);
+        U[3] = 0.5 * U[0] * (ic.mach * ic.mach / (r * r)) + pr * (1. / (ic.gamma - 1.));
+        return;
+    }
+    const double PI = 3.14159265358979323846;
+    const double g = ic.gamma;
+    double du = 0, dv = 0, dT = 0;
+    for (int k = 0; k < ic.nv; k++) {
+        double dx = wrap_d(x - ic.vx[k], ic.Lx), dy = wrap_d(y - ic.vy[k], ic.Ly);
+        double r2 = dx * dx + dy * dy;
+        double ex = exp(0.5 * (1. - r2));
+        double beta = ic.beta[k];
+        du += -beta / (2. * PI) * dy * ex;
+        dv += beta / (2. * PI) * dx * ex;
+        dT += -(g - 1.) * beta * beta / (8. * g * PI * PI) * ex * ex;
+    }
+    double T_inf = ic.p_inf / ic.rho_inf;
+    double T = T_inf + dT;
+    double rho = ic.rho_inf * pow(T / T_inf, 1. / (g - 1.));
+    double pp = ic.p_inf * pow(rho / ic.rho_inf, g);
+    double uu = ic.u_inf + du, vv = ic.v_inf + dv;
+    U[0] = rho;
+    U[1] = rho * uu;
+    U[2] = rho * vv;
+    U[3] = pp / (g - 1.) + 0.5 * rho * (uu * uu + vv * vv);
+}
 
 hipError_t upload_tables();
 hipError_t upload_tables_kernels(const TabAll<double>& h64, const TabAll<float>& h32);

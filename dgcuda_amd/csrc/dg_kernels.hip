@@ -79,41 +79,6 @@ __global__ void k_set_dt(Ctrl* ctrl, double dt, int advance) {
 // Initial conditions: c_i = sum_j w_j U0(x_j) phi_i(x_j) in FP64
 // (conserv_kernels.cu:167-187, euler.cu:31-59), stored in the bulk precision.
 // ---------------------------------------------------------------------------
-__device__ inline double wrap_d(double d, double L) { return (L > 0) ? d - L * rint(d / L) : d; }
-
-__device__ void ic_point(const IcP& ic, double x, double y, double* U) {
-    if (ic.kind == 0) {
-        for (int n = 0; n < 4; n++) U[n] = ic.constant[n];
-        return;
-    }
-    if (ic.kind == 1) {
-        double dx = wrap_d(x - ic.gx, ic.Lx), dy = wrap_d(y - ic.gy, ic.Ly);
-        U[0] = exp(-ic.gk * (dx * dx + dy * dy));
-        return;
-    }
-    const double PI = 3.14159265358979323846;
-    const double g = ic.gamma;
-    double du = 0, dv = 0, dT = 0;
-    for (int k = 0; k < ic.nv; k++) {
-        double dx = wrap_d(x - ic.vx[k], ic.Lx), dy = wrap_d(y - ic.vy[k], ic.Ly);
-        double r2 = dx * dx + dy * dy;
-        double ex = exp(0.5 * (1. - r2));
-        double beta = ic.beta[k];
-        du += -beta / (2. * PI) * dy * ex;
-        dv += beta / (2. * PI) * dx * ex;
-        dT += -(g - 1.) * beta * beta / (8. * g * PI * PI) * ex * ex;
-    }
-    double T_inf = ic.p_inf / ic.rho_inf;
-    double T = T_inf + dT;
-    double rho = ic.rho_inf * pow(T / T_inf, 1. / (g - 1.));
-    double p = ic.p_inf * pow(rho / ic.rho_inf, g);
-    double uu = ic.u_inf + du, vv = ic.v_inf + dv;
-    U[0] = rho;
-    U[1] = rho * uu;
-    U[2] = rho * vv;
-    U[3] = p / (g - 1.) + 0.5 * rho * (uu * uu + vv * vv);
-}
-
 template <int N, int P, typename R>
 __global__ void __launch_bounds__(128) k_project_ic(R* c, StateIx ix, const double* V, int K, IcP ic) {
     constexpr int NP = n_modes(P), NQ = n_quad_vol(P);

@@ -169,7 +169,58 @@ IcP make_ic(const dg_problem_desc& d, double Lx, double Ly) {
         ic.vy[k] = d.vortex_y[k];
         ic.constant[k] = d.constant[k];
     }
+    ic.mach = d.mach;
+    ic.fpow = (d.quirks & DG_QUIRK_FLOAT_POW) ? 1 : 0;
     return ic;
+}
+
+// Per boundary side and face point, 4 FP64 values for the lane kernel
+// (boundary_ghost): inflow and outflow (euler.cu:177-241) take the problem's
+// state U0 at the point; reflecting (supersonic.cu:64-108) the unit radial
+// direction there, turned to the side's normal, as the wall normal.  The
+// point is the left element's face point j mapped by the reference's
+// formula x = x2 r + x3 s + x1 (1 - r - s): face 0 (0.5 + 0.5 r_j, 0), face
+// 1 ((1 - r_j)/2, (1 + r_j)/2), face 2 (0, 0.5 + 0.5 r_{n-1-j}); the
+// reflecting side maps face 1 as ((1 + r_j)/2, (1 - r_j)/2) under
+// DG_QUIRK_REFLECT_MIRROR, as the reference does, and takes the radial
+// length through a float square root under DG_QUIRK_FLOAT_SQRT.
+std::vector<double> boundary_data(const ElemGeometry& g, const Tables& T, const IcP& ic, int quirks) {
+    const int n1 = T.n_q1;
+    std::vector<double> out((size_t)g.bfaces.size() * n1 * 4, 0.0);
+    for (size_t b = 0; b < g.bfaces.size(); b++) {
+        const int e = g.bfaces[b].elem, f = g.bfaces[b].face, kind = g.bfaces[b].kind;
+        const double v1x = g.V[0][e], v1y = g.V[1][e], v2x = g.V[2][e], v2y = g.V[3][e];
+        const double v3x = g.V[4][e], v3y = g.V[5][e];
+        const bool mirror = kind == BC_REFLECTING && (quirks & DG_QUIRK_REFLECT_MIRROR);
+        for (int j = 0; j < n1; j++) {
+            double r1, r2;
+            switch (f) {
+                case 0: r1 = 0.5 + 0.5 * T.s_r[j]; r2 = 0.; break;
+                case 1:
+                    if (mirror) { r1 = (1. + T.s_r[j]) / 2.; r2 = (1. - T.s_r[j]) / 2.; }
+                    else { r1 = (1. - T.s_r[j]) / 2.; r2 = (1. + T.s_r[j]) / 2.; }
+                    break;
+                default: r1 = 0.; r2 = 0.5 + 0.5 * T.s_r[n1 - 1 - j]; break;
+            }
+            const double x = v2x * r1 + v3x * r2 + v1x * (1 - r1 - r2);
+            const double y = v2y * r1 + v3y * r2 + v1y * (1 - r1 - r2);
+            double* o = &out[(b * n1 + j) * 4];
+            if (kind == BC_REFLECTING) {
+                const double d2 = x * x + y * y;
+                const double dot = (quirks & DG_QUIRK_FLOAT_SQRT) ? (double)std::sqrt((float)d2) : std::sqrt(d2);
+                double Nx = x / dot, Ny = y / dot;
+                if (Nx * g.fnx[f][e] + Ny * g.fny[f][e] < 0) {
+                    Nx *= -1;
+                    Ny *= -1;
+                }
+                o[0] = Nx;
+                o[1] = Ny;
+            } else {
+                ic_point(ic, x, y, o);
+            }
+        }
+    }
+    return out;
 }
 
 Status check_problem(const dg_problem_desc& d, int p, int precision) {
@@ -177,7 +228,9 @@ Status check_problem(const dg_problem_desc& d, int p, int precision) {
     if (precision != DG_FP64 && precision != DG_FP32) return Status::err(DG_ERR_ARG, "precision must be DG_FP64 or DG_FP32");
     if (d.physics != DG_PHYS_ADVECTION && d.physics != DG_PHYS_EULER)
         return Status::err(DG_ERR_ARG, "physics must be DG_PHYS_ADVECTION or DG_PHYS_EULER");
-    if (d.ic < 0 || d.ic > 2) return Status::err(DG_ERR_ARG, "unknown initial condition");
+    if (d.ic < 0 || d.ic > 3) return Status::err(DG_ERR_ARG, "unknown initial condition");
+    if (d.ic == DG_IC_SUPERSONIC && (d.physics != DG_PHYS_EULER || !(d.mach > 0.0)))
+        return Status::err(DG_ERR_ARG, "supersonic vortex needs Euler physics and a positive Mach number");
     if (d.physics == DG_PHYS_ADVECTION && d.ic == DG_IC_VORTEX)
         return Status::err(DG_ERR_ARG, "vortex initial condition needs Euler physics");
     if (d.physics == DG_PHYS_EULER && d.ic == DG_IC_GAUSSIAN)
@@ -185,7 +238,7 @@ Status check_problem(const dg_problem_desc& d, int p, int precision) {
     if (d.ic == DG_IC_VORTEX && (d.n_vortex < 1 || d.n_vortex > DG_MAX_VORTEX))
         return Status::err(DG_ERR_ARG, "n_vortex must be in 1..4");
     if (d.physics == DG_PHYS_EULER && !(d.gamma > 1.0)) return Status::err(DG_ERR_ARG, "gamma must exceed 1");
-    if (d.quirks & ~3) return Status::err(DG_ERR_ARG, "unknown quirk bits");
+    if (d.quirks & ~15) return Status::err(DG_ERR_ARG, "unknown quirk bits");
     return Status::ok();
 }
 
@@ -216,6 +269,8 @@ class SolverT : public Solver {
     int64_t tile_bytes = 0;  // fword, halo and packed geometry
     MfmaArgs mx{};
     double *V = nullptr, *Jd = nullptr, *audit_part = nullptr;
+    double* bnd = nullptr;      // boundary sides' ghost data (boundary_data)
+    bool has_boundary = false;  // the lane kernel only
     Ctrl* ctrl = nullptr;
     Ctrl hctrl{};
     LaunchShape shape;
@@ -235,7 +290,7 @@ class SolverT : public Solver {
     ~SolverT() override {
         if (stream) (void)hipStreamSynchronize(stream);  // nothing queued may use the buffers
         if (gexec) (void)hipGraphExecDestroy(gexec);
-        void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, audit_part, ctrl, fword, halo, face4, metric4};
+        void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, audit_part, ctrl, fword, halo, face4, metric4, bnd};
         for (void* b : bufs) pool_free(b);
         pool_release_stream(stream);
     }
@@ -281,6 +336,9 @@ class SolverT : public Solver {
         ElemGeometry g;
         std::string why = elem_geometry(m, perm, pr.quirks, g);
         if (!why.empty()) return Status::err(DG_ERR_UNSUPPORTED, why);
+        has_boundary = !g.bfaces.empty();
+        if (has_boundary && p > VALU_BOUNDARY_MAX_P)
+            return Status::err(DG_ERR_UNSUPPORTED, "boundary sides are supported up to p = 3");
         min_r = g.min_r;
         Status st = ensure_device_ready();
         if (!st) return st;
@@ -327,6 +385,11 @@ class SolverT : public Solver {
         HIPST(hipMemcpyAsync(geo_f, hf.data(), hf.size() * sizeof(R), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(nbr, hn.data(), hn.size() * sizeof(int), hipMemcpyHostToDevice, stream));
         for (int f = 0; f < 3; f++) h_nbr[f] = g.nbr[f];
+        if (has_boundary) {
+            const std::vector<double> hb = boundary_data(g, build_tables(p), ic, pr.quirks);
+            if (!(st = alloc((void**)&bnd, hb.size() * sizeof(double)))) return st;
+            HIPST(hipMemcpyAsync(bnd, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        }
         HIPST(hipMemcpyAsync(V, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(Jd, hj.data(), hj.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         hctrl = Ctrl{};
@@ -406,6 +469,8 @@ class SolverT : public Solver {
         const bool mf = mfma_supported(N, p);
         if ((kind == 2 || kind == 3) && !mf)
             return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs p = 1..4");
+        if ((kind == 2 || kind == 3) && has_boundary)
+            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs a mesh without boundary sides");
         if (kind < 1 || kind > 3) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1, 2 or 3");
         s = LaunchShape{};
         s.kind = kind;
@@ -551,6 +616,7 @@ class SolverT : public Solver {
         a.ntiles = shape.ntiles;
         a.xcd_remap = shape.xcd_remap;
         a.phys = phys;
+        a.bnd = bnd;
         hipError_t e;
         hipStream_t sx = on ? on : stream;
         if (shape.kind >= 2)

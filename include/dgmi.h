@@ -49,7 +49,7 @@ typedef enum {
     DG_ERR_IO = 2,          /* file could not be read or written */
     DG_ERR_PARSE = 3,       /* malformed mesh file */
     DG_ERR_HIP = 4,         /* HIP runtime error or no GPU */
-    DG_ERR_UNSUPPORTED = 5, /* valid but not supported (e.g. boundary sides) */
+    DG_ERR_UNSUPPORTED = 5, /* valid but not supported (e.g. a boundary mesh in a batch) */
     DG_ERR_NUMERIC = 6,     /* non-finite state or non-positive density / pressure */
     DG_ERR_BUDGET = 7,      /* request exceeds the memory budget */
     DG_ERR_STATE = 8        /* call out of order */
@@ -61,14 +61,23 @@ enum { DG_PHYS_ADVECTION = 1, DG_PHYS_EULER = 4 };
 enum { DG_FP64 = 0, DG_FP32 = 1 };
 /* time integrators, numbered as the reference's RK4 / RK2 (conserv.cu:19-20) */
 enum { DG_RK4 = 1, DG_RK2 = 2 };
-/* initial conditions */
-enum { DG_IC_CONSTANT = 0, DG_IC_GAUSSIAN = 1, DG_IC_VORTEX = 2 };
-/* optional reproduction of the reference's float roundings:
- *  FLOAT_SQRT: sound speed and cell wave speed through a float square root
- *              (euler.cu:80, 269);
- *  FLOAT_GEOM: side lengths, normals and inscribed diameters in float
- *              (conserv_kernels.cu:346, 364-371, 434). */
-enum { DG_QUIRK_FLOAT_SQRT = 1, DG_QUIRK_FLOAT_GEOM = 2 };
+/* initial conditions; DG_IC_SUPERSONIC is the supersonic vortex between two
+ * arcs (euler/supersonic/supersonic.cu:28-38), also the state its inflow and
+ * outflow boundaries impose */
+enum { DG_IC_CONSTANT = 0, DG_IC_GAUSSIAN = 1, DG_IC_VORTEX = 2, DG_IC_SUPERSONIC = 3 };
+/* optional reproduction of the reference's float roundings and quirks:
+ *  FLOAT_SQRT:     sound speed, cell wave speed and the reflecting wall's
+ *                  radial length through a float square root (euler.cu:80,
+ *                  269; supersonic.cu:99);
+ *  FLOAT_GEOM:     side lengths, normals and inscribed diameters in float
+ *                  (conserv_kernels.cu:346, 364-371, 434);
+ *  FLOAT_POW:      the supersonic vortex's powers through powf
+ *                  (supersonic.cu:31, 35);
+ *  REFLECT_MIRROR: the reflecting boundary maps face 1's point j to the
+ *                  mirrored point of the side (supersonic.cu:86-89), where
+ *                  inflow and outflow use the trace's own point
+ *                  (euler.cu:192-195). */
+enum { DG_QUIRK_FLOAT_SQRT = 1, DG_QUIRK_FLOAT_GEOM = 2, DG_QUIRK_FLOAT_POW = 4, DG_QUIRK_REFLECT_MIRROR = 8 };
 
 #define DG_MAX_VORTEX 4
 
@@ -85,6 +94,7 @@ typedef struct {
     double rho_inf, u_inf, v_inf, p_inf; /* DG_IC_VORTEX background state */
     double vortex_beta[DG_MAX_VORTEX], vortex_x[DG_MAX_VORTEX], vortex_y[DG_MAX_VORTEX];
     double constant[4]; /* DG_IC_CONSTANT: conserved state */
+    double mach;        /* DG_IC_SUPERSONIC: Mach number at the inner arc r = 1 (the reference's 2.25) */
 } dg_problem_desc;
 
 typedef struct {
@@ -136,7 +146,13 @@ int dg_tables(int p, int* sizes, double* r1, double* r2, double* w, double* s_r,
               double* gx, double* gy, double* side, double* vertex);
 
 /* ---------------- one solve on the GPU ---------------- */
-/* p in 0..5 as the reference's -n option; precision DG_FP64 / DG_FP32 */
+/* p in 0..5 as the reference's -n option; precision DG_FP64 / DG_FP32.
+ * Boundary sides (right element -1 reflecting, -2 outflow, -3 inflow, as the
+ * reference's eval_left_right, conserv_kernels.cu:647-671) take ghost states:
+ * inflow and outflow the problem's initial state at the face point
+ * (euler.cu:177-241), reflecting the interior density and energy and the
+ * velocity's component along the wall whose normal is the radial direction
+ * (supersonic.cu:64-108).  Such meshes run on the lane-per-element kernel. */
 int dg_solver_create(const dg_mesh* m, const dg_problem_desc* prob, int p, int precision, dg_solver** out);
 void dg_solver_destroy(dg_solver* s);
 /* sizes[0..5] = num_elem, num_sides, n_p, n_q, n_q1, N */
@@ -147,8 +163,8 @@ int dg_project_ic(dg_solver* s);
 int dg_set_state(dg_solver* s, const double* c_ref);
 int dg_get_state(dg_solver* s, double* c_ref);
 /* k = dt * M^-1 R(u) exactly as eval_surface + eval_volume + eval_rhs compute it
- * (M = J*I); u and k are host buffers in the reference layout; t is unused on
- * periodic meshes and kept for the reference's signature */
+ * (M = J*I); u and k are host buffers in the reference layout; t is kept for
+ * the reference's signature (the boundary states do not depend on it) */
 int dg_rhs(dg_solver* s, const double* u_ref, double* k_ref, double t, double dt);
 /* one RK step of size dt on the resident state (scheme DG_RK4 / DG_RK2) */
 int dg_rk_step(dg_solver* s, double dt, int scheme);

@@ -51,6 +51,7 @@ def lib():
             "or_tables": (None, [VP] + [dp] * 10),
             "or_geometry": (D, [VP, dp, dp]),
             "or_project_ic": (None, [VP, dp, dp]),
+            "or_set_bc_ic": (None, [VP, dp, I]),
             "or_ic_point": (None, [dp, I, D, D, D, dp]),
             "or_rhs": (I, [VP, dp, dp, D]),
             "or_rhs_parts": (I, [VP, dp, dp, dp, dp]),
@@ -94,12 +95,22 @@ def ic_poly(alpha):
     return np.array([3.0, float(alpha)])
 
 
+def ic_point(ic, N, gamma, x, y):
+    """The oracle's U0(x, y) for an IC parameter array."""
+    ic = np.ascontiguousarray(ic, np.float64)
+    U = np.zeros(4)
+    lib().or_ic_point(_dp(ic), N, gamma, x, y, _dp(U))
+    return U[:N]
+
+
 def ic_array(prob, period=(0.0, 0.0)):
     """Flatten a dgcuda_amd.ProblemDesc into the oracle's IC parameter array."""
     if prob.ic == 0:
         return np.array([0.0] + list(prob.constant))
     if prob.ic == 1:
         return np.array([1.0, prob.gauss_x, prob.gauss_y, prob.gauss_k, period[0], period[1]])
+    if prob.ic == 3:  # supersonic vortex: {4, mach, float_pow}
+        return np.array([4.0, prob.mach, 1.0 if prob.quirks & 4 else 0.0])
     a = [2.0, float(prob.n_vortex), prob.rho_inf, prob.u_inf, prob.v_inf, prob.p_inf, period[0], period[1]]
     for k in range(prob.n_vortex):
         a += [prob.vortex_beta[k], prob.vortex_x[k], prob.vortex_y[k]]
@@ -128,7 +139,15 @@ class Oracle:
     @classmethod
     def from_mesh(cls, mesh, p, prob, hoist=1):
         V, es, sd, si = mesh.arrays()
-        return cls(V, es, sd, si, p, prob.physics, prob.gamma, (prob.ax, prob.ay), prob.quirks, hoist)
+        o = cls(V, es, sd, si, p, prob.physics, prob.gamma, (prob.ax, prob.ay), prob.quirks, hoist)
+        if (np.asarray(si).reshape(4, -1)[1] < 0).any():  # boundary sides: inflow / outflow states
+            o.set_bc_ic(ic_array(prob, mesh.period))
+        return o
+
+    def set_bc_ic(self, ic):
+        """IC parameter array (as for project_ic) of the inflow / outflow states."""
+        self._bc_ic = np.ascontiguousarray(ic, np.float64)
+        lib().or_set_bc_ic(self._h, _dp(self._bc_ic), self._bc_ic.size)
 
     @property
     def shape(self):
@@ -161,7 +180,7 @@ class Oracle:
         u = np.ascontiguousarray(u, np.float64)
         k = np.zeros(self.shape)
         if lib().or_rhs(self._h, _dp(u), _dp(k), dt) != 0:
-            raise ValueError("oracle: mesh has boundary sides")
+            raise ValueError("oracle: mesh has boundary sides and no inflow / outflow state (set_bc_ic)")
         return k
 
     def rhs_parts(self, u):
@@ -169,7 +188,7 @@ class Oracle:
         q = np.zeros(self.shape)
         l, r = np.zeros((self.N, self.n_p, self.S)), np.zeros((self.N, self.n_p, self.S))
         if lib().or_rhs_parts(self._h, _dp(u), _dp(q), _dp(l), _dp(r)) != 0:
-            raise ValueError("oracle: mesh has boundary sides")
+            raise ValueError("oracle: mesh has boundary sides and no inflow / outflow state (set_bc_ic)")
         return q, l, r
 
     def cell_lambda(self, u):
@@ -185,7 +204,7 @@ class Oracle:
     def rk_step(self, u, dt, scheme=1):
         u = np.array(u, np.float64, copy=True)
         if lib().or_rk_step(self._h, _dp(u), dt, scheme) != 0:
-            raise ValueError("oracle: mesh has boundary sides")
+            raise ValueError("oracle: mesh has boundary sides and no inflow / outflow state (set_bc_ic)")
         return u
 
     def integrate(self, u, t_end, cfl, scheme=1):
@@ -193,7 +212,7 @@ class Oracle:
         t = C.c_double()
         n = lib().or_integrate(self._h, _dp(u), t_end, cfl, scheme, C.byref(t))
         if n < 0:
-            raise ValueError("oracle: mesh has boundary sides")
+            raise ValueError("oracle: mesh has boundary sides and no inflow / outflow state (set_bc_ic)")
         return u, n, t.value
 
     def run_steps(self, u, nsteps, cfl, scheme=1):

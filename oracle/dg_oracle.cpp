@@ -30,6 +30,9 @@
  *   src/2d/conservation/time_integrator.cu:194-434     rk_tempstorage, rk4, time_integrate_rk4
  *   src/2d/conservation/time_integrator.cu:445-614     rk2, time_integrate_rk2
  *   src/2d/conservation/euler/euler.cu:252-278        eval_global_lambda
+ *   src/2d/conservation/euler/euler.cu:177-241        inflow_boundary, outflow_boundary
+ *   src/2d/conservation/euler/supersonic/supersonic.cu:28-108  U0, U_inflow, U_outflow,
+ *                                                     reflecting_boundary (supersonic vortex)
  *   src/2d/conservation/conserv_kernels.cu:818-841    eval_u (vertex values)
  *
  * Built with -O2 -ffp-contract=off and no fast-math (no FMA contraction, so each
@@ -47,8 +50,10 @@
 namespace {
 
 enum { PHYS_ADVECTION = 1, PHYS_EULER = 4 };
-enum { QUIRK_FLOAT_SQRT = 1, QUIRK_FLOAT_GEOM = 2 };
-enum { IC_CONSTANT = 0, IC_GAUSSIAN = 1, IC_VORTEX = 2, IC_POLY = 3 };
+enum { QUIRK_FLOAT_SQRT = 1, QUIRK_FLOAT_GEOM = 2, QUIRK_FLOAT_POW = 4, QUIRK_REFLECT_MIRROR = 8 };
+enum { IC_CONSTANT = 0, IC_GAUSSIAN = 1, IC_VORTEX = 2, IC_POLY = 3, IC_SUPERSONIC = 4 };
+/* boundary side codes in right_elem (conserv_kernels.cu:647-671) */
+enum { BC_REFLECTING = -1, BC_OUTFLOW = -2, BC_INFLOW = -3 };
 
 const double PI = 3.14159265358979323846;
 
@@ -105,7 +110,8 @@ inline double fsqrt_quirk(double x) { return (double)sqrtf((float)x); }
 struct Ctx {
     int K = 0, S = 0, p = 0, n_p = 0, n_q = 0, n_q1 = 0, N = 0;
     int physics = 0, quirks = 0, hoist = 1;
-    bool has_boundary = false;  // boundary sides: projection / output only
+    bool has_boundary = false;  // boundary sides: the ghost states need bc_ic
+    std::vector<double> bc_ic;  // IC parameters of the inflow / outflow states
     double gamma = 1.4, ax = 1.0, ay = 1.0;
     std::vector<double> V1x, V1y, V2x, V2y, V3x, V3y;
     std::vector<int> es1, es2, es3, left_elem, right_elem, lsn, rsn;
@@ -264,12 +270,62 @@ void trace(const Ctx& c, const double* C, int face, int j, double* U) {
         for (int n = 0; n < N; n++) U[n] += C[n * np + i] * c.side[face * np * n1 + i * n1 + j];
 }
 
+void ic_point(const double* ic, int N, double gamma, double x, double y, double* U);
+
+/* physical point of the left element's face point j (inflow_boundary,
+ * euler.cu:186-204): face 0 (0.5 + 0.5 r_j, 0), face 1 ((1 - r_j)/2,
+ * (1 + r_j)/2), face 2 (0, 0.5 + 0.5 r_{n1-1-j}); x = x2 r + x3 s + x1 (1 - r - s).
+ * `mirror` takes face 1 as ((1 + r_j)/2, (1 - r_j)/2), as reflecting_boundary
+ * does (supersonic.cu:86-89). */
+void face_point(const Ctx& c, int l, int ls, int j, bool mirror, double* x, double* y) {
+    const int n1 = c.n_q1;
+    double r1, r2;
+    switch (ls) {
+        case 0: r1 = 0.5 + 0.5 * c.s_r[j]; r2 = 0.; break;
+        case 1:
+            if (mirror) { r1 = (1. + c.s_r[j]) / 2.; r2 = (1. - c.s_r[j]) / 2.; }
+            else { r1 = (1. - c.s_r[j]) / 2.; r2 = (1. + c.s_r[j]) / 2.; }
+            break;
+        default: r1 = 0.; r2 = 0.5 + 0.5 * c.s_r[n1 - 1 - j]; break;
+    }
+    *x = c.V2x[l] * r1 + c.V3x[l] * r2 + c.V1x[l] * (1 - r1 - r2);
+    *y = c.V2y[l] * r1 + c.V3y[l] * r2 + c.V1y[l] * (1 - r1 - r2);
+}
+
+/* ghost state of a boundary side (eval_left_right, conserv_kernels.cu:647-671):
+ * reflecting keeps rho and E and the velocity's component along the wall,
+ * whose normal is the radial direction at the point (supersonic.cu:64-108);
+ * inflow and outflow take the problem's state there (euler.cu:177-241). */
+void ghost_state(const Ctx& c, int code, int l, int ls, int j, double nx, double ny, const double* UL, double* UR) {
+    double x, y;
+    if (code == BC_REFLECTING) {
+        face_point(c, l, ls, j, (c.quirks & QUIRK_REFLECT_MIRROR) != 0, &x, &y);
+        UR[0] = UL[0];
+        if (c.N == 1) return;
+        UR[3] = UL[3];
+        double dot = (c.quirks & QUIRK_FLOAT_SQRT) ? fsqrt_quirk(x * x + y * y) : sqrt(x * x + y * y);
+        double Nx = x / dot, Ny = y / dot;
+        if (Nx * nx + Ny * ny < 0) {
+            Nx *= -1;
+            Ny *= -1;
+        }
+        UR[1] = (UL[1] * Ny - UL[2] * Nx) * Ny;
+        UR[2] = -(UL[1] * Ny - UL[2] * Nx) * Nx;
+        return;
+    }
+    face_point(c, l, ls, j, false, &x, &y);
+    ic_point(c.bc_ic.data(), c.N, c.gamma, x, y, UR);
+}
+
 void riemann_point(const Ctx& c, const double* CL, const double* CR, int ls, int rs, int j,
-                   double nx, double ny, double* s_out) {
+                   double nx, double ny, double* s_out, int l = 0, int code = 0) {
     const int N = c.N, n1 = c.n_q1;
-    double UL[4], UR[4], fxl[4], fyl[4], fxr[4], fyr[4];
+    double UL[4] = {0, 0, 0, 0}, UR[4] = {0, 0, 0, 0}, fxl[4], fyl[4], fxr[4], fyr[4];
     trace(c, CL, ls, j, UL);
-    trace(c, CR, rs, n1 - j - 1, UR);
+    if (code < 0)
+        ghost_state(c, code, l, ls, j, nx, ny, UL, UR);
+    else
+        trace(c, CR, rs, n1 - j - 1, UR);
     eval_flux(c, UL, fxl, fyl);
     eval_flux(c, UR, fxr, fyr);
     double lambda = eval_lambda(c, UL, UR, nx, ny);
@@ -284,13 +340,16 @@ void eval_surface(Ctx& c, const double* u) {
         int l = c.left_elem[idx], r = c.right_elem[idx];
         int ls = c.lsn[idx], rs = c.rsn[idx];
         double nx = c.Nx[idx], ny = c.Ny[idx], len = c.len[idx];
+        const int code = r < 0 ? r : 0;
         for (int i = 0; i < np; i++)
             for (int n = 0; n < N; n++) {
                 CL[n * np + i] = u[K * np * n + i * K + l];
-                CR[n * np + i] = u[K * np * n + i * K + r];
+                CR[n * np + i] = r >= 0 ? u[K * np * n + i * K + r] : 0.;
             }
+        if (r < 0) rs = 0;
         if (c.hoist)
-            for (int j = 0; j < n1; j++) riemann_point(c, CL.data(), CR.data(), ls, rs, j, nx, ny, &sj[j * 4]);
+            for (int j = 0; j < n1; j++)
+                riemann_point(c, CL.data(), CR.data(), ls, rs, j, nx, ny, &sj[j * 4], l, code);
         for (int i = 0; i < np; i++) {
             double sum_left[4] = {0, 0, 0, 0}, sum_right[4] = {0, 0, 0, 0};
             for (int j = 0; j < n1; j++) {
@@ -299,7 +358,7 @@ void eval_surface(Ctx& c, const double* u) {
                 if (c.hoist) {
                     s = &sj[j * 4];
                 } else {
-                    riemann_point(c, CL.data(), CR.data(), ls, rs, j, nx, ny, s_loc);
+                    riemann_point(c, CL.data(), CR.data(), ls, rs, j, nx, ny, s_loc, l, code);
                     s = s_loc;
                 }
                 for (int n = 0; n < N; n++) {
@@ -447,6 +506,24 @@ void ic_point(const double* ic, int N, double gamma, double x, double y, double*
         U[0] = exp(-ic[3] * (dx * dx + dy * dy));
         return;
     }
+    if (kind == IC_SUPERSONIC) {
+        /* supersonic vortex (supersonic.cu:28-38): ic = {4, mach, float_pow}.
+         * float_pow rounds each power's arguments and result to float, as the
+         * reference's powf on double arguments does; otherwise FP64 pow. */
+        const double mach = ic[1];
+        const bool fp = ic[2] != 0.;
+        auto pw = [fp](double a, double b) {
+            return fp ? (double)(float)pow((double)(float)a, (double)(float)b) : pow(a, b);
+        };
+        double r = sqrt(x * x + y * y);
+        U[0] = pw(1 + 1.0125 * (1. - 1. / (r * r)), 2.5);
+        if (N == 1) return;
+        U[1] = U[0] * sin(atan(y / x)) * mach / r;
+        U[2] = U[0] * -cos(atan(y / x)) * mach / r;
+        double p = (1.0 / gamma) * pw(U[0], gamma);
+        U[3] = 0.5 * U[0] * (mach * mach / (r * r)) + p * (1. / (gamma - 1.));
+        return;
+    }
     /* isentropic vortex superposition: ic = {2, nv, rho_inf, u_inf, v_inf, p_inf, Lx, Ly, (beta, xc, yc) x nv} */
     int nv = (int)ic[1];
     double rho_inf = ic[2], u_inf = ic[3], v_inf = ic[4], p_inf = ic[5], Lx = ic[6], Ly = ic[7];
@@ -490,6 +567,9 @@ void project_ic(const Ctx& c, const double* ic, double* u) {
     }
 }
 
+/* a mesh with boundary sides needs the inflow / outflow IC */
+bool bc_ready(const Ctx& c) { return !c.has_boundary || !c.bc_ic.empty(); }
+
 }  // namespace
 
 extern "C" {
@@ -499,10 +579,10 @@ extern "C" {
  *   elem_s[3*K] = elem_s1 | elem_s2 | elem_s3
  *   sides[4*S]  = sides_x1 | sides_y1 | sides_x2 | sides_y2
  *   side_int[4*S] = left_elem | right_elem | left_side_number | right_side_number
- * phys = {gamma, ax, ay}.  Returns NULL for p outside 0..5.  Meshes with
- * boundary sides (right_elem < 0) are accepted for projection and output
- * only: the boundary conditions are not part of this path, and or_rhs /
- * or_rk_step / or_integrate refuse them. */
+ * phys = {gamma, ax, ay}.  Returns NULL for p outside 0..5.  Boundary sides
+ * (right_elem -1 reflecting, -2 outflow, -3 inflow) take ghost states; the
+ * inflow / outflow states come from the IC given to or_set_bc_ic, without
+ * which or_rhs / or_rk_step / or_integrate refuse such a mesh. */
 void* or_create(int K, int S, const double* V, const int* elem_s, const double* sides, const int* side_int,
                 int p, int physics, const double* phys, int quirks, int hoist) {
     if (p < 0 || p > 5 || (physics != PHYS_ADVECTION && physics != PHYS_EULER)) return nullptr;
@@ -531,6 +611,9 @@ void* or_create(int K, int S, const double* V, const int* elem_s, const double* 
 }
 
 void or_destroy(void* h) { delete (Ctx*)h; }
+
+/* IC parameters (the or_project_ic array) of the inflow / outflow states */
+void or_set_bc_ic(void* h, const double* ic, int n) { ((Ctx*)h)->bc_ic.assign(ic, ic + n); }
 
 void or_set_hoist(void* h, int hoist) { ((Ctx*)h)->hoist = hoist; }
 
@@ -580,7 +663,7 @@ void or_ic_point(const double* ic, int N, double gamma, double x, double y, doub
 
 /* k = dt * M^-1 R(u) with M = J*I, in the reference layout [var][mode][elem] */
 int or_rhs(void* h, const double* u, double* k, double dt) {
-    if (((Ctx*)h)->has_boundary) return -1;
+    if (!bc_ready(*(Ctx*)h)) return -1;
     rhs(*(Ctx*)h, u, k, dt);
     return 0;
 }
@@ -588,7 +671,7 @@ int or_rhs(void* h, const double* u, double* k, double dt) {
 /* reference-side split of the RHS: quad_rhs [N*n_p*K], left/right side arrays [N*n_p*S] */
 int or_rhs_parts(void* h, const double* u, double* quad, double* left, double* right) {
     Ctx* c = (Ctx*)h;
-    if (c->has_boundary) return -1;
+    if (!bc_ready(*c)) return -1;
     eval_surface(*c, u);
     eval_volume(*c, u);
     size_t M = (size_t)c->N * c->n_p * c->K, MS = (size_t)c->N * c->n_p * c->S;
@@ -606,7 +689,7 @@ void or_cell_lambda(void* h, const double* u, double* lam) {
 double or_cfl_dt(void* h, const double* u, double cfl) { return cfl_dt(*(Ctx*)h, u, cfl); }
 
 int or_rk_step(void* h, double* u, double dt, int scheme) {
-    if (((Ctx*)h)->has_boundary) return -1;
+    if (!bc_ready(*(Ctx*)h)) return -1;
     rk_step(*(Ctx*)h, u, dt, scheme);
     return 0;
 }
@@ -615,7 +698,7 @@ int or_rk_step(void* h, double* u, double dt, int scheme) {
  * advanced before the stages, last step clamped to t_end.  Returns steps. */
 int or_integrate(void* h, double* u, double t_end, double cfl, int scheme, double* t_out) {
     Ctx* c = (Ctx*)h;
-    if (c->has_boundary) return -1;
+    if (!bc_ready(*c)) return -1;
     double t = 0;
     int steps = 0;
     while (t < t_end) {
@@ -636,7 +719,7 @@ int or_integrate(void* h, double* u, double t_end, double cfl, int scheme, doubl
 /* nsteps CFL steps without an end time; returns the final time */
 double or_run_steps(void* h, double* u, int nsteps, double cfl, int scheme) {
     Ctx* c = (Ctx*)h;
-    if (c->has_boundary) return -1.0;
+    if (!bc_ready(*c)) return -1.0;
     double t = 0;
     for (int s = 0; s < nsteps; s++) {
         double dt = cfl_dt(*c, u, cfl);

@@ -11,8 +11,9 @@ fixtures it writes are committed, so the tests never read the reference.
                     src/2d/tests/t0/alpha{a}_{mesh}{p}.out for the meshes
                     canonical, box and supersimple, plus those meshes
                     (src/2d/mesh/*.pmsh, advection-prototype 6-field dialect)
-  sv1.pmsh          conservation-dialect mesh with boundary sides
-                    (src/2d/conservation/euler/supersonic/mesh/sv1.pmsh)
+  sv1.pmsh          conservation-dialect meshes with boundary sides (the
+  sv1refined.pmsh   supersonic vortex's annulus sector and its two
+  sv1refined1.pmsh  refinements; src/2d/conservation/euler/supersonic/mesh/)
 """
 import json
 import math
@@ -76,8 +77,8 @@ def main():
             for p in range(a, 6):
                 name = "alpha%d_%s%d.out" % (a, mesh, p)
                 shutil.copy(os.path.join(src2d, "tests", "t0", name), os.path.join(t0, name))
-    shutil.copy(os.path.join(src2d, "conservation", "euler", "supersonic", "mesh", "sv1.pmsh"),
-                os.path.join(OUT, "sv1.pmsh"))
+    for name in ("sv1.pmsh", "sv1refined.pmsh", "sv1refined1.pmsh"):
+        shutil.copy(os.path.join(src2d, "conservation", "euler", "supersonic", "mesh", name), os.path.join(OUT, name))
     print("fixtures written to", OUT)
 
 

@@ -1,4 +1,6 @@
 """Shared helpers of the test suite (no GPU needed to import)."""
+import os
+
 import numpy as np
 
 import dgcuda_amd as dg
@@ -49,3 +51,25 @@ def shuffled_mesh(n, seed=7, lo=0.0, hi=10.0):
 
 def rel_max(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+GOLD_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden_mesh(name):
+    """A shipped reference mesh kept under tests/golden."""
+    return dg.Mesh.read_pmsh(os.path.join(GOLD_DIR, name + ".pmsh"))
+
+
+def relabelled(m):
+    """The same mesh with every element's vertices relabelled (V2, V3, V1):
+    local face f becomes face (f + 2) % 3 (boundary sides on face 2 move to
+    face 1)."""
+    V, es, sd, si = m.arrays()
+    si = si.copy()
+    Vn = np.vstack([V[2], V[3], V[4], V[5], V[0], V[1]])
+    esn = np.vstack([es[1], es[2], es[0]])
+    si[2] = (si[2] + 2) % 3
+    inner = si[1] >= 0
+    si[3][inner] = (si[3][inner] + 2) % 3
+    return dg.Mesh.from_arrays(Vn, esn, sd, si, m.period)

@@ -131,8 +131,8 @@ def test_pmsh_roundtrip(tmp_path):
 
 def test_read_reference_boundary_mesh():
     """The reference's supersonic-vortex mesh (conservation dialect, with
-    boundary sides): read, cross-referenced, and refused by the solver,
-    which handles periodic meshes only."""
+    boundary sides): read and cross-referenced; the solver accepts it and,
+    with no GPU here, fails with DG_ERR_HIP (there is no CPU fallback)."""
     m = dg.Mesh.read_pmsh(os.path.join(GOLD, "sv1.pmsh"))
     assert m.num_elem > 0 and m.num_boundary_sides > 0
     V, es, sd, si = m.arrays()
@@ -140,7 +140,11 @@ def test_read_reference_boundary_mesh():
     J = (V[2] - V[0]) * (V[5] - V[1]) - (V[4] - V[0]) * (V[3] - V[1])
     assert np.all(J != 0)
     with pytest.raises(dg.DGError) as e:
-        dg.Solver(m, dg.euler_vortex(), 1)
+        dg.Solver(m, dg.supersonic_vortex(), 1)
+    assert e.value.code == dg.STATUS_CODES["DG_ERR_HIP"]
+    # boundary sides are supported up to p = 3, refused before any device work
+    with pytest.raises(dg.DGError) as e:
+        dg.Solver(m, dg.supersonic_vortex(), 4)
     assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
 
 
@@ -192,7 +196,9 @@ def test_geometry_bitwise_vs_oracle(quirks):
     (dict(prob=dg.problem(dg.DG_PHYS_ADVECTION, dg.DG_IC_VORTEX, vortices=((1.0, 0.0, 0.0),))), "DG_ERR_ARG"),
     (dict(prob=dg.problem(dg.DG_PHYS_EULER, dg.DG_IC_VORTEX)), "DG_ERR_ARG"),  # no vortex given
     (dict(prob=dg.euler_vortex(gamma=1.0)), "DG_ERR_ARG"),
-    (dict(prob=dg.euler_vortex(quirks=4)), "DG_ERR_ARG"),
+    (dict(prob=dg.euler_vortex(quirks=16)), "DG_ERR_ARG"),
+    (dict(prob=dg.supersonic_vortex(mach=0.0)), "DG_ERR_ARG"),
+    (dict(prob=dg.problem(dg.DG_PHYS_ADVECTION, dg.DG_IC_SUPERSONIC)), "DG_ERR_ARG"),
 ])
 def test_solver_argument_errors(kw, code):
     m = dg.Mesh.periodic_jittered(4, 4, 0.0, 1.0, 0.0, 1.0, 0.0, 1)
