@@ -108,6 +108,8 @@ def lib():
             "dg_time_stage_kernels": (I, [VP, I, D, dp]), "dg_time_step_graph": (I, [VP, I, D, dp]),
             "dg_solver_device_bytes": (I, [VP, P(I64)]),
             "dg_bandwidth_probe": (I, [I64, I, dp]), "dg_selftest_mfma": (I, [dp]),
+            "dg_write_msh": (I, [VP, I, dp, C.c_char_p]),
+            "dg_l2_error": (I, [VP, P(ProblemDesc), I, dp, dp]),
             "dg_batch_create": (I, [I, P(VP)]), "dg_batch_destroy": (None, [VP]),
             "dg_batch_add": (I, [VP, VP, P(ProblemDesc), I, D, ip]), "dg_batch_project_ic": (I, [VP]),
             "dg_batch_run_steps": (I, [VP, I, D, I]), "dg_batch_integrate": (I, [VP, D, I, P(I64)]),
@@ -223,6 +225,18 @@ class Mesh:
 
     def write_pmsh(self, path):
         _check(lib().dg_mesh_write_pmsh(self._h, str(path).encode()))
+
+    def write_msh(self, var, uv, path):
+        """gmsh view of vertex values uv[K, 3] as the reference writes output/U<var>.msh."""
+        uv = np.ascontiguousarray(uv, np.float64).reshape(-1)
+        _check(lib().dg_write_msh(self._h, var, _dp(uv), str(path).encode()))
+
+    def l2_error(self, prob, p, c):
+        """Per-variable L2 error of a state c[N, n_p, K] against the problem's U0."""
+        c = np.ascontiguousarray(c, np.float64)
+        err = np.zeros(prob.physics)
+        _check(lib().dg_l2_error(self._h, C.byref(prob), p, _dp(c), _dp(err)))
+        return err
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

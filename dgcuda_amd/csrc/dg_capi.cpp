@@ -181,6 +181,61 @@ int dg_mesh_geometry(const dg_mesh* mh, int quirks, double* geo_e, double* geo_s
 
 void dg_mesh_destroy(dg_mesh* m) { delete m; }
 
+int dg_write_msh(const dg_mesh* mh, int var, const double* uv, const char* path) {
+    GUARD_BEGIN
+    if (!mh || !uv || !path) return fail(DG_ERR_ARG, "null argument");
+    const Mesh& m = mh->m;
+    FILE* f = fopen(path, "w");
+    if (!f) return fail(DG_ERR_IO, std::string("cannot open ") + path + " for writing");
+    fprintf(f, "View \"U%i \" {\n", var);
+    for (int i = 0; i < m.K; i++)
+        fprintf(f, "ST (%lf,%lf,0,%lf,%lf,0,%lf,%lf,0) {%lf,%lf,%lf};\n", m.V1x[i], m.V1y[i], m.V2x[i], m.V2y[i],
+                m.V3x[i], m.V3y[i], uv[3 * (size_t)i], uv[3 * (size_t)i + 1], uv[3 * (size_t)i + 2]);
+    fprintf(f, "};");
+    const bool bad = ferror(f) != 0;
+    fclose(f);
+    if (bad) return fail(DG_ERR_IO, std::string("write error on ") + path);
+    return DG_OK;
+    GUARD_END
+}
+
+int dg_l2_error(const dg_mesh* mh, const dg_problem_desc* prob, int p, const double* c, double* err) {
+    GUARD_BEGIN
+    if (!mh || !prob || !c || !err) return fail(DG_ERR_ARG, "null argument");
+    Status st = check_problem(*prob, p, DG_FP64);
+    if (!st) return fail(st.code, st.msg);
+    const Mesh& m = mh->m;
+    // a rule of higher degree than the projection's: with as many points as
+    // modes (p <= 2) the projection interpolates at its own rule's points.
+    // The graded basis is hierarchical, so the first n_p rows of the higher
+    // order's table are this order's basis.
+    const Tables T = build_tables(std::min(p + 2, 5));
+    const IcP ic = make_ic(*prob, m.period_x, m.period_y);
+    const int K = m.K, N = prob->physics, np = n_modes(p), nq = T.n_q;
+    std::vector<double> ge, gs;
+    mesh_geometry(m, prob->quirks, ge, gs);
+    std::vector<long double> acc(N, 0.0L);
+    for (int e = 0; e < K; e++) {
+        const double J = ge[e];
+        for (int q = 0; q < nq; q++) {
+            const double r1 = T.r1[q], r2 = T.r2[q];
+            const double x = r1 * m.V2x[e] + r2 * m.V3x[e] + (1 - r1 - r2) * m.V1x[e];
+            const double y = r1 * m.V2y[e] + r2 * m.V3y[e] + (1 - r1 - r2) * m.V1y[e];
+            double U[4];
+            ic_point(ic, x, y, U);
+            for (int v = 0; v < N; v++) {
+                double uh = 0;
+                for (int i = 0; i < np; i++) uh += c[((size_t)v * np + i) * K + e] * T.basis[(size_t)i * nq + q];
+                const double d = uh - U[v];
+                acc[v] += (long double)(J * T.w[q] * d * d);
+            }
+        }
+    }
+    for (int v = 0; v < N; v++) err[v] = std::sqrt((double)acc[v]);
+    return DG_OK;
+    GUARD_END
+}
+
 int dg_tables(int p, int* sizes, double* r1, double* r2, double* w, double* s_r, double* w1, double* basis, double* gx,
               double* gy, double* side, double* vertex) {
     GUARD_BEGIN

@@ -139,6 +139,20 @@ int dg_mesh_get_arrays(const dg_mesh* m, double* V, int* elem_s, double* sides, 
 int dg_mesh_geometry(const dg_mesh* m, int quirks, double* geo_e, double* geo_s, double* min_r);
 void dg_mesh_destroy(dg_mesh* m);
 
+/* gmsh view of one variable's vertex values as the reference writes
+ * output/U<var>.msh (conservation/main.cu:187-209): "View \"U<var> \" {",
+ * one "ST (x1,y1,0,x2,y2,0,x3,y3,0) {u1,u2,u3};" line per element in the
+ * mesh's order (%lf), "};".  uv[3*K] as dg_eval_vertices returns it. */
+int dg_write_msh(const dg_mesh* m, int var, const double* uv, const char* path);
+
+/* L2 error of a state (reference layout, FP64, the mesh's element order)
+ * against the problem's initial condition U0, per variable:
+ * err[v] = sqrt(sum_K J_K sum_q w_q (u_h(x_q) - U0(x_q))^2) with the volume
+ * rule of order min(p + 2, 5) (degree 2 min(p + 2, 5)).  The exact solution where U0 is steady (the
+ * supersonic vortex) or after whole periods of a periodic flow; the
+ * reference's eval_error is a stub.  Host only. */
+int dg_l2_error(const dg_mesh* m, const dg_problem_desc* prob, int p, const double* c_ref, double* err);
+
 /* FP64 basis and quadrature tables for order p in the reference's preval_basis
  * layouts (basis.cu:150-226): r1,r2,w[n_q]; s_r,w1[n_q1]; basis,gx,gy[n_p*n_q];
  * side[3*n_p*n_q1]; vertex[3*n_p].  sizes[0..2] = n_p, n_q, n_q1. */

@@ -14,6 +14,8 @@ fixtures it writes are committed, so the tests never read the reference.
   sv1.pmsh          conservation-dialect meshes with boundary sides (the
   sv1refined.pmsh   supersonic vortex's annulus sector and its two
   sv1refined1.pmsh  refinements; src/2d/conservation/euler/supersonic/mesh/)
+  euler_U0.msh      the reference's gmsh output of density on sv1refined1
+                    (src/2d/conservation/euler/output/U0.msh)
 """
 import json
 import math
@@ -72,13 +74,14 @@ def main():
     t0 = os.path.join(OUT, "t0")
     os.makedirs(t0, exist_ok=True)
     for mesh in ("canonical", "box", "supersimple"):
-        shutil.copy(os.path.join(src2d, "mesh", mesh + ".pmsh"), os.path.join(t0, mesh + ".pmsh"))
+        shutil.copyfile(os.path.join(src2d, "mesh", mesh + ".pmsh"), os.path.join(t0, mesh + ".pmsh"))
         for a in range(4):
             for p in range(a, 6):
                 name = "alpha%d_%s%d.out" % (a, mesh, p)
-                shutil.copy(os.path.join(src2d, "tests", "t0", name), os.path.join(t0, name))
+                shutil.copyfile(os.path.join(src2d, "tests", "t0", name), os.path.join(t0, name))
     for name in ("sv1.pmsh", "sv1refined.pmsh", "sv1refined1.pmsh"):
-        shutil.copy(os.path.join(src2d, "conservation", "euler", "supersonic", "mesh", name), os.path.join(OUT, name))
+        shutil.copyfile(os.path.join(src2d, "conservation", "euler", "supersonic", "mesh", name), os.path.join(OUT, name))
+    shutil.copyfile(os.path.join(src2d, "conservation", "euler", "output", "U0.msh"), os.path.join(OUT, "euler_U0.msh"))
     print("fixtures written to", OUT)
 
 
