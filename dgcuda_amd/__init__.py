@@ -89,6 +89,7 @@ def lib():
         sig = {
             "dg_last_error": (C.c_char_p, []), "dg_version": (C.c_char_p, []), "dg_device_count": (I, []),
             "dg_mesh_read_pmsh": (I, [C.c_char_p, P(VP)]),
+            "dg_mesh_read_gmsh": (I, [C.c_char_p, I, P(VP)]),
             "dg_mesh_from_arrays": (I, [I, I, dp, ip, dp, ip, D, D, P(VP)]),
             "dg_mesh_periodic_jittered": (I, [I, I, D, D, D, D, D, C.c_uint64, P(VP)]),
             "dg_mesh_write_pmsh": (I, [VP, C.c_char_p]),
@@ -189,6 +190,14 @@ class Mesh:
     def read_pmsh(cls, path):
         h = C.c_void_p()
         _check(lib().dg_mesh_read_pmsh(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read_gmsh(cls, path, genmesh_quirks=False):
+        """gmsh 2.2 .msh converted as the reference's genmesh.py does (with
+        genmesh_quirks, exactly, including its latent bugs)."""
+        h = C.c_void_p()
+        _check(lib().dg_mesh_read_gmsh(str(path).encode(), 1 if genmesh_quirks else 0, C.byref(h)))
         return cls(h)
 
     @classmethod

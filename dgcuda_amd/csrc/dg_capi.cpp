@@ -79,6 +79,20 @@ int dg_mesh_read_pmsh(const char* path, dg_mesh** out) {
     GUARD_END
 }
 
+int dg_mesh_read_gmsh(const char* path, int genmesh_quirks, dg_mesh** out) {
+    GUARD_BEGIN
+    if (!path || !out) return fail(DG_ERR_ARG, "null argument");
+    auto* h = new dg_mesh();
+    std::string why = read_gmsh(path, h->m, genmesh_quirks != 0);
+    if (!why.empty()) {
+        delete h;
+        return fail(why.rfind("cannot open", 0) == 0 ? DG_ERR_IO : DG_ERR_PARSE, why);
+    }
+    *out = h;
+    return DG_OK;
+    GUARD_END
+}
+
 int dg_mesh_from_arrays(int K, int S, const double* V, const int* elem_s, const double* sides, const int* side_int,
                         double period_x, double period_y, dg_mesh** out) {
     GUARD_BEGIN

@@ -1,8 +1,11 @@
 #include "dg_mesh.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -52,6 +55,149 @@ std::string read_pmsh(const char* path, Mesh& m) {
     fclose(f);
     m.period_x = m.period_y = 0;
     return validate(m);
+}
+
+std::string read_gmsh(const char* path, Mesh& m, bool quirks) {
+    FILE* f = fopen(path, "r");
+    if (!f) return std::string("cannot open mesh file ") + path;
+    char line[4096];
+    auto fail = [&](const std::string& why) {
+        fclose(f);
+        return why;
+    };
+    auto seek = [&](const char* tag) {
+        while (fgets(line, sizeof line, f))
+            if (strstr(line, tag)) return true;
+        return false;
+    };
+    if (!seek("$Nodes")) return fail("no $Nodes section");
+    int nv = 0;
+    if (!fgets(line, sizeof line, f) || sscanf(line, "%d", &nv) != 1 || nv <= 0) return fail("bad node count");
+    std::vector<double> vx(nv), vy(nv);
+    for (int i = 0; i < nv; i++) {
+        int id;
+        if (!fgets(line, sizeof line, f) || sscanf(line, "%d %lf %lf", &id, &vx[i], &vy[i]) != 3)
+            return fail("bad node line " + std::to_string(i));
+    }
+    if (!seek("$Elements")) return fail("no $Elements section");
+    int ne = 0;
+    if (!fgets(line, sizeof line, f) || sscanf(line, "%d", &ne) != 1 || ne < 0) return fail("bad element count");
+    std::vector<std::array<int, 3>> tri;
+    std::vector<std::pair<int, int>> bside;
+    std::vector<int> btag;
+    for (int i = 0; i < ne; i++) {
+        if (!fgets(line, sizeof line, f)) return fail("unexpected end of file in $Elements");
+        long t[16];
+        int n = 0;
+        for (char* tok = strtok(line, " \t\r\n"); tok && n < 16; tok = strtok(nullptr, " \t\r\n")) t[n++] = atol(tok);
+        // genmesh.py:44-58: 7 fields = a tagged line, 8 fields = a triangle
+        if (n == 7) {
+            bside.push_back({(int)t[5] - 1, (int)t[6] - 1});
+            btag.push_back((int)t[3]);
+        } else if (n == 8) {
+            tri.push_back({(int)t[5] - 1, (int)t[6] - 1, (int)t[7] - 1});
+        }
+    }
+    fclose(f);
+    const int K = (int)tri.size();
+    if (K == 0) return "mesh has no triangles";
+    for (auto& e : tri)
+        for (int v : e)
+            if (v < 0 || v >= nv) return "element refers to a missing node";
+    // boundary code of a vertex pair, either orientation (the last tagged line wins)
+    std::map<std::pair<int, int>, int> bcode;
+    for (size_t j = 0; j < bside.size(); j++) {
+        const int code = btag[j] == 10000 ? -1 : btag[j] == 20000 ? -2 : btag[j] == 30000 ? -3 : 0;
+        if (code == 0) continue;
+        bcode[bside[j]] = code;
+        bcode[{bside[j].second, bside[j].first}] = code;
+    }
+    std::vector<int> left_elem(3 * (size_t)K, 0), right_elem(3 * (size_t)K, 0), lsn(3 * (size_t)K, 0),
+        rsn(3 * (size_t)K, 0);
+    std::vector<std::array<int, 3>> es(K, {0, 0, 0});
+    std::vector<std::pair<int, int>> sidelist(3 * (size_t)K);
+    std::vector<char> has_right(3 * (size_t)K, 0);
+    std::map<std::pair<int, int>, int> sidemap;
+    int ns = 0;
+    for (int i = 0; i < K; i++) {
+        auto& e = tri[i];
+        const double J = (vx[e[1]] - vx[e[0]]) * (vy[e[2]] - vy[e[0]]) - (vx[e[2]] - vx[e[0]]) * (vy[e[1]] - vy[e[0]]);
+        if (J < 0) {
+            if (quirks) {  // genmesh.py:100-103 swaps the shared coordinates
+                std::swap(vx[e[0]], vx[e[1]]);
+                std::swap(vy[e[0]], vy[e[1]]);
+            } else {
+                std::swap(e[0], e[1]);
+            }
+        }
+        const std::pair<int, int> fs[3] = {{e[0], e[1]}, {e[1], e[2]}, {e[2], e[0]}};
+        bool is_new[3] = {true, true, true};
+        for (int k = 0; k < 3; k++)
+            for (const auto& key : {fs[k], std::make_pair(fs[k].second, fs[k].first)}) {
+                auto it = sidemap.find(key);
+                if (it == sidemap.end()) continue;
+                const int j = it->second;
+                is_new[k] = false;
+                right_elem[j] = i;
+                has_right[j] = 1;
+                es[i][k] = j;
+                rsn[j] = k;
+            }
+        for (int k = 0; k < 3; k++) {
+            if (!is_new[k]) continue;
+            sidemap[fs[k]] = ns;
+            sidelist[ns] = fs[k];
+            lsn[ns] = k;
+            auto b = bcode.find(fs[k]);
+            if (b != bcode.end()) right_elem[ns] = b->second;
+            es[i][k] = ns;
+            left_elem[ns] = i;
+            ns++;
+        }
+    }
+    if (!quirks)
+        for (int s = 0; s < ns; s++)
+            if (!has_right[s] && right_elem[s] >= 0)
+                return "side " + std::to_string(s) + " has one element and no boundary tag";
+    // boundary sides to the front, reflecting then outflow then inflow
+    // (genmesh.py:236-283: the swap of sides i and j and its fix-ups)
+    auto slot = [&](int elem) -> std::array<int, 3>* {
+        if (elem >= 0) return &es[elem];
+        if (quirks && elem >= -3 && K + elem >= 0) return &es[K + elem];  // negative index
+        return nullptr;
+    };
+    int j = 0;
+    for (int code : {-1, -2, -3}) {
+        for (int i = 0; i < ns; i++) {
+            if (right_elem[i] != code) continue;
+            if (auto* a = slot(left_elem[j])) (*a)[lsn[j]] = i;
+            if (auto* a = slot(right_elem[j])) (*a)[rsn[j]] = i;
+            if (auto* a = slot(left_elem[i])) (*a)[lsn[i]] = j;
+            std::swap(sidelist[i], sidelist[j]);
+            std::swap(left_elem[i], left_elem[j]);
+            std::swap(right_elem[i], right_elem[j]);
+            std::swap(lsn[i], lsn[j]);
+            std::swap(rsn[i], rsn[j]);
+            j++;
+        }
+    }
+    m = Mesh{};
+    m.K = K;
+    m.S = ns;
+    for (int i = 0; i < K; i++) {
+        const auto& e = tri[i];
+        m.V1x.push_back(vx[e[0]]); m.V1y.push_back(vy[e[0]]);
+        m.V2x.push_back(vx[e[1]]); m.V2y.push_back(vy[e[1]]);
+        m.V3x.push_back(vx[e[2]]); m.V3y.push_back(vy[e[2]]);
+        m.elem_s1.push_back(es[i][0]); m.elem_s2.push_back(es[i][1]); m.elem_s3.push_back(es[i][2]);
+    }
+    for (int s = 0; s < ns; s++) {
+        m.sx1.push_back(vx[sidelist[s].first]); m.sy1.push_back(vy[sidelist[s].first]);
+        m.sx2.push_back(vx[sidelist[s].second]); m.sy2.push_back(vy[sidelist[s].second]);
+        m.left_elem.push_back(left_elem[s]); m.right_elem.push_back(right_elem[s]);
+        m.left_side.push_back(lsn[s]); m.right_side.push_back(rsn[s]);
+    }
+    return quirks ? std::string() : validate(m);
 }
 
 std::string write_pmsh(const char* path, const Mesh& m) {

@@ -28,6 +28,22 @@ struct Mesh {
 
 // Returns an empty string on success, otherwise the reason.
 std::string read_pmsh(const char* path, Mesh& m);
+
+// gmsh 2.2 ASCII mesh (.msh) to the side structure, as the reference's
+// converter builds it (src/2d/genmesh.py:11-310): triangles (8-field element
+// lines) in file order; sides numbered as first met over (element, local
+// face 0..2); the element that first meets a side is its left element;
+// 2-node line elements tagged 10000 / 20000 / 30000 mark reflecting /
+// outflow / inflow sides (right element -1 / -2 / -3); boundary sides are
+// then moved to the front, reflecting first.  With `genmesh_quirks` the
+// converter's two latent bugs are reproduced, so shipped .msh/.pmsh pairs
+// convert exactly: an element with negative Jacobian swaps the coordinates
+// of its first two vertices in the shared vertex list, and the front-sort
+// writes the slot of a boundary side's "right element" through a negative
+// index (element K + code).  Without it the element's own vertex order is
+// swapped, the sort leaves boundary sides' missing right elements alone,
+// and a side that belongs to one element but carries no tag is an error.
+std::string read_gmsh(const char* path, Mesh& m, bool genmesh_quirks);
 std::string write_pmsh(const char* path, const Mesh& m);
 
 // n_x * n_y squares on [x0,x1] x [y0,y1], periodic in both directions, each

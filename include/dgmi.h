@@ -120,6 +120,17 @@ int dg_device_count(void);
 /* ---------------- meshes (host only) ---------------- */
 /* reads the conservation-framework .pmsh dialect (conserv.cu:118-192) */
 int dg_mesh_read_pmsh(const char* path, dg_mesh** out);
+/* gmsh 2.2 ASCII .msh converted as the reference's genmesh.py (src/2d/genmesh.py:11-310)
+ * builds a .pmsh: sides numbered as first met over (element, face 0..2), the
+ * first element meeting a side is its left element, line elements tagged
+ * 10000 / 20000 / 30000 make reflecting / outflow / inflow sides, boundary
+ * sides sorted to the front.  genmesh_quirks = 1 reproduces the converter
+ * exactly, including its two latent bugs (a negative-Jacobian element swaps
+ * shared vertex coordinates; the front-sort writes through a negative
+ * element index); 0 swaps the element's own vertices instead, leaves
+ * boundary sides' missing right elements alone and rejects untagged open
+ * sides. */
+int dg_mesh_read_gmsh(const char* path, int genmesh_quirks, dg_mesh** out);
 /* builds a mesh from the arrays read_mesh returns, in its order:
  * V[6K] = V1x|V1y|V2x|V2y|V3x|V3y, elem_s[3K] = elem_s1|elem_s2|elem_s3,
  * sides[4S] = x1|y1|x2|y2, side_int[4S] = left_elem|right_elem|left_side_number|right_side_number.

@@ -16,6 +16,9 @@ fixtures it writes are committed, so the tests never read the reference.
   sv1refined1.pmsh  refinements; src/2d/conservation/euler/supersonic/mesh/)
   euler_U0.msh      the reference's gmsh output of density on sv1refined1
                     (src/2d/conservation/euler/output/U0.msh)
+  gmsh/             .msh / .pmsh pairs the reference's genmesh.py made:
+                    sv1, sv1refined (supersonic/mesh), cylinder, cylinder1
+                    (cylinder/mesh)
 """
 import json
 import math
@@ -82,6 +85,13 @@ def main():
     for name in ("sv1.pmsh", "sv1refined.pmsh", "sv1refined1.pmsh"):
         shutil.copyfile(os.path.join(src2d, "conservation", "euler", "supersonic", "mesh", name), os.path.join(OUT, name))
     shutil.copyfile(os.path.join(src2d, "conservation", "euler", "output", "U0.msh"), os.path.join(OUT, "euler_U0.msh"))
+    gm = os.path.join(OUT, "gmsh")
+    os.makedirs(gm, exist_ok=True)
+    for sub, name in (("supersonic", "sv1"), ("supersonic", "sv1refined"), ("cylinder", "cylinder"),
+                      ("cylinder", "cylinder1")):
+        for ext in (".msh", ".pmsh"):
+            shutil.copyfile(os.path.join(src2d, "conservation", "euler", sub, "mesh", name + ext),
+                            os.path.join(gm, name + ext))
     print("fixtures written to", OUT)
 
 
