@@ -385,8 +385,10 @@ class SolverT : public Solver {
         HIPST(hipMemcpyAsync(geo_f, hf.data(), hf.size() * sizeof(R), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(nbr, hn.data(), hn.size() * sizeof(int), hipMemcpyHostToDevice, stream));
         for (int f = 0; f < 3; f++) h_nbr[f] = g.nbr[f];
+        // host source of an async copy: lives until the stream is synchronised below
+        std::vector<double> hb;
         if (has_boundary) {
-            const std::vector<double> hb = boundary_data(g, build_tables(p), ic, pr.quirks);
+            hb = boundary_data(g, build_tables(p), ic, pr.quirks);
             if (!(st = alloc((void**)&bnd, hb.size() * sizeof(double)))) return st;
             HIPST(hipMemcpyAsync(bnd, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         }
