@@ -327,39 +327,47 @@ __device__ __forceinline__ R llf_speed(const PhysP& ph, const R* UL, const R* UR
     }
 }
 
-// FP32 Euler face terms of one side: the normal flux F(U).n and the wave
-// speed |u.n| + c of the local Lax-Friedrichs flux (euler.cu flux and
-// eval_lambda), with the hardware reciprocal and square root as in flux and
-// llf_speed.  F.n is formed directly (u.n first) instead of from the two
-// Cartesian fluxes, 17 operations per side.
-__device__ __forceinline__ void normal_flux_speed(const PhysP& ph, const float* U, float nx, float ny, float* F,
-                                                  float& spd) {
-    const float g = (float)ph.gamma;
-    const float rho = U[0], ru = U[1], rv = U[2], E = U[3];
-    const float ir = __builtin_amdgcn_rcpf(rho);
-    const float u = ru * ir, v = rv * ir;
-    const float p = (g - 1.f) * (E - 0.5f * (ru * u + rv * v));
-    const float un = u * nx + v * ny;
+// Reciprocal and the sound speed's square root of the normal-flux forms:
+// FP32 the hardware instructions (about 1 ulp), FP64 IEEE division and
+// square root (a float square root under DG_QUIRK_FLOAT_SQRT, euler.cu:80).
+__device__ __forceinline__ float nf_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double nf_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float nf_sqrt(float x, int) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double nf_sqrt(double x, int quirks) { return qsqrt<double>(x, quirks & 1); }
+
+// Euler face terms of one side: the normal flux F(U).n and the wave speed
+// |u.n| + c of the local Lax-Friedrichs flux (euler.cu flux and
+// eval_lambda).  F.n is formed directly (u.n first) instead of from the two
+// Cartesian fluxes, with one reciprocal of rho: 17 operations per side
+// where the reference's form divides by rho seven times.
+template <typename R>
+__device__ __forceinline__ void normal_flux_speed(const PhysP& ph, const R* U, R nx, R ny, R* F, R& spd) {
+    const R g = (R)ph.gamma;
+    const R rho = U[0], ru = U[1], rv = U[2], E = U[3];
+    const R ir = nf_rcp(rho);
+    const R u = ru * ir, v = rv * ir;
+    const R p = (g - R(1)) * (E - R(0.5) * (ru * u + rv * v));
+    const R un = u * nx + v * ny;
     F[0] = ru * nx + rv * ny;
     F[1] = ru * un + p * nx;
     F[2] = rv * un + p * ny;
     F[3] = (E + p) * un;
-    spd = fabsf(un) + __builtin_amdgcn_sqrtf(g * p * ir);
+    spd = fabs(un) + nf_sqrt(g * p * ir, ph.quirks);
 }
 
-// FP32 Euler volume terms of one quadrature point: the fluxes contracted
-// with the two rows of the element's inverse map, Fr = F.(ys, -xs) and
+// Euler volume terms of one quadrature point: the fluxes contracted with the
+// two rows of the element's inverse map, Fr = F.(ys, -xs) and
 // Fs = F.(-yr, xr) (the F.n form of normal_flux_speed for both directions,
 // sharing u, v, p and E + p).
-__device__ __forceinline__ void metric_flux(const PhysP& ph, const float* U, float xr, float yr, float xs, float ys,
-                                            float* Fr, float* Fs) {
-    const float g = (float)ph.gamma;
-    const float rho = U[0], ru = U[1], rv = U[2], E = U[3];
-    const float ir = __builtin_amdgcn_rcpf(rho);
-    const float u = ru * ir, v = rv * ir;
-    const float p = (g - 1.f) * (E - 0.5f * (ru * u + rv * v));
-    const float Ep = E + p;
-    const float u1 = u * ys - v * xs, u2 = v * xr - u * yr;
+template <typename R>
+__device__ __forceinline__ void metric_flux(const PhysP& ph, const R* U, R xr, R yr, R xs, R ys, R* Fr, R* Fs) {
+    const R g = (R)ph.gamma;
+    const R rho = U[0], ru = U[1], rv = U[2], E = U[3];
+    const R ir = nf_rcp(rho);
+    const R u = ru * ir, v = rv * ir;
+    const R p = (g - R(1)) * (E - R(0.5) * (ru * u + rv * v));
+    const R Ep = E + p;
+    const R u1 = u * ys - v * xs, u2 = v * xr - u * yr;
     Fr[0] = ru * ys - rv * xs;
     Fr[1] = ru * u1 + p * ys;
     Fr[2] = rv * u1 - p * xs;

@@ -428,8 +428,8 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
         for (int v = 0; v < N; v++) Uq[v] = U[v][i];
         const bool qv = M::pos(g, i) < NQ;
         if (N == 4 && !qv) Uq[0] = R(1);  // padding point: keep the flux finite
-        if constexpr (sizeof(R) == 4 && N == 4) {
-            metric_flux(a.phys, Uq, xr, yr, xs, ys, fx, fy);  // fx, fy hold Fr, Fs
+        if constexpr (N == 4) {
+            metric_flux<R>(a.phys, Uq, xr, yr, xs, ys, fx, fy);  // fx, fy hold Fr, Fs
 #pragma unroll
             for (int v = 0; v < N; v++) {
                 Rv[v] = M::mma(Agx[i], qv ? fx[v] : R(0), Rv[v]);
@@ -459,7 +459,7 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
             const bool left = (wd >> 14) & 1;
             const R* own = ts + el * TS + pt * N;
             const R* nbt = ts + slot * TS + (fn * NQ1 + NQ1 - 1 - jo) * N;
-            if constexpr (sizeof(R) == 4 && N == 4) {
+            if constexpr (N == 4) {
                 // F(UL).n + F(UR).n, max of the two speeds and UL - UR with the
                 // sign of this side folded into the speed: sums and maxima of
                 // the two sides' terms commute and the sign flips are exact, so
@@ -471,9 +471,9 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
                     uo[v] = own[v];
                     un[v] = nbt[v];
                 }
-                normal_flux_speed(a.phys, uo, r[0], r[1], Fo, so);
-                normal_flux_speed(a.phys, un, r[0], r[1], Fb, sb);
-                const R lm = fmaxf(so, sb);
+                normal_flux_speed<R>(a.phys, uo, r[0], r[1], Fo, so);
+                normal_flux_speed<R>(a.phys, un, r[0], r[1], Fb, sb);
+                const R lm = fmax(so, sb);
                 const R lms = left ? lm : -lm;
                 const R cf = left ? R(-0.25) * r[2] : R(0.25) * r[2];
 #pragma unroll
