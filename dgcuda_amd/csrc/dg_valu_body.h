@@ -54,6 +54,9 @@ __device__ __forceinline__ void boundary_ghost(const double* bnd, int w, int j, 
 template <int N, int P, typename R>
 __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int kind) {
     constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1;
+    // FP32 Euler up to p = 3 uses the normal-flux forms of the matrix-core
+    // kernels (fewer operations; p >= 4 keeps its register allocation)
+    constexpr bool NF_FORM = sizeof(R) == 4 && N == 4 && P <= 3;
     const TabP<R, P>& T = tab<R, P>();
     const bool valid = e < a.K;
     const int ee = valid ? e : a.K - 1;
@@ -84,12 +87,25 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
                 for (int v = 0; v < N; v++) U[v] += u[v][m] * T.B[m][q];
             R fx[N], fy[N];
-            flux<N, R>(a.phys, U, fx, fy);
+            if constexpr (NF_FORM) {
+                // FP32 Euler: fluxes contracted with the inverse map first
+                // (metric_flux), then with the weighted basis gradients
+                metric_flux<R>(a.phys, U, xr, yr, xs, ys, fx, fy);
 #pragma unroll
-            for (int m = 0; m < NP; m++) {
-                const R bgx = T.Gx[m][q], bgy = T.Gy[m][q];
+                for (int m = 0; m < NP; m++) {
+                    const R bgx = T.Gx[m][q], bgy = T.Gy[m][q];
 #pragma unroll
-                for (int v = 0; v < N; v++) tot[v][m] += fx[v] * (bgx * ys - bgy * yr) + fy[v] * (-bgx * xs + bgy * xr);
+                    for (int v = 0; v < N; v++) tot[v][m] += fx[v] * bgx + fy[v] * bgy;
+                }
+            } else {
+                flux<N, R>(a.phys, U, fx, fy);
+#pragma unroll
+                for (int m = 0; m < NP; m++) {
+                    const R bgx = T.Gx[m][q], bgy = T.Gy[m][q];
+#pragma unroll
+                    for (int v = 0; v < N; v++)
+                        tot[v][m] += fx[v] * (bgx * ys - bgy * yr) + fy[v] * (-bgx * xs + bgy * xr);
+                }
             }
         }
     }
@@ -136,19 +152,33 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
             }
             if constexpr (P <= VALU_BOUNDARY_MAX_P)
                 if (bnd) boundary_ghost<N, NQ1, R>(a.bnd, w, j, Uo, Un);
-            R UL[N], UR[N];
+            if constexpr (NF_FORM) {
+                // FP32 Euler: normal fluxes of both sides, the side's sign in
+                // the speed (the matrix-core kernel's form; both elements of
+                // the side still get bit-identical values)
+                R Fo[N], Fb[N], so, sb;
+                normal_flux_speed<R>(a.phys, Uo, nx, ny, Fo, so);
+                normal_flux_speed<R>(a.phys, Un, nx, ny, Fb, sb);
+                const R lm = fmax(so, sb);
+                const R lms = left ? lm : -lm;
 #pragma unroll
-            for (int v = 0; v < N; v++) {
-                UL[v] = left ? Uo[v] : Un[v];
-                UR[v] = left ? Un[v] : Uo[v];
+                for (int v = 0; v < N; v++) FS[j][v] = T.W1[j] * (R(0.5) * fma(lms, Uo[v] - Un[v], Fo[v] + Fb[v]));
+            } else {
+                R UL[N], UR[N];
+#pragma unroll
+                for (int v = 0; v < N; v++) {
+                    UL[v] = left ? Uo[v] : Un[v];
+                    UR[v] = left ? Un[v] : Uo[v];
+                }
+                R fxl[N], fyl[N], fxr[N], fyr[N];
+                flux<N, R>(a.phys, UL, fxl, fyl);
+                flux<N, R>(a.phys, UR, fxr, fyr);
+                const R lam = llf_speed<N, R>(a.phys, UL, UR, nx, ny);
+#pragma unroll
+                for (int v = 0; v < N; v++)
+                    FS[j][v] = T.W1[j] *
+                               (R(0.5) * ((fxl[v] + fxr[v]) * nx + (fyl[v] + fyr[v]) * ny + lam * (UL[v] - UR[v])));
             }
-            R fxl[N], fyl[N], fxr[N], fyr[N];
-            flux<N, R>(a.phys, UL, fxl, fyl);
-            flux<N, R>(a.phys, UR, fxr, fyr);
-            const R lam = llf_speed<N, R>(a.phys, UL, UR, nx, ny);
-#pragma unroll
-            for (int v = 0; v < N; v++)
-                FS[j][v] = T.W1[j] * (R(0.5) * ((fxl[v] + fxr[v]) * nx + (fyl[v] + fyr[v]) * ny + lam * (UL[v] - UR[v])));
         }
         const R coef = left ? -len / R(2) : len / R(2);
 #pragma unroll
