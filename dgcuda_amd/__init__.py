@@ -29,6 +29,7 @@ DG_FP64, DG_FP32 = 0, 1
 DG_RK4, DG_RK2 = 1, 2
 DG_IC_CONSTANT, DG_IC_GAUSSIAN, DG_IC_VORTEX, DG_IC_SUPERSONIC = 0, 1, 2, 3
 DG_QUIRK_FLOAT_SQRT, DG_QUIRK_FLOAT_GEOM, DG_QUIRK_FLOAT_POW, DG_QUIRK_REFLECT_MIRROR = 1, 2, 4, 8
+DG_GMSH_QUIRKS, DG_GMSH_PERIODIC = 1, 2
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_IO", 3: "DG_ERR_PARSE", 4: "DG_ERR_HIP",
                 5: "DG_ERR_UNSUPPORTED", 6: "DG_ERR_NUMERIC", 7: "DG_ERR_BUDGET", 8: "DG_ERR_STATE"}
 STATUS_CODES = {v: k for k, v in STATUS_NAMES.items()}
@@ -90,6 +91,7 @@ def lib():
             "dg_last_error": (C.c_char_p, []), "dg_version": (C.c_char_p, []), "dg_device_count": (I, []),
             "dg_mesh_read_pmsh": (I, [C.c_char_p, P(VP)]),
             "dg_mesh_read_gmsh": (I, [C.c_char_p, I, P(VP)]),
+            "dg_mesh_period": (I, [VP, P(D), P(D)]),
             "dg_mesh_from_arrays": (I, [I, I, dp, ip, dp, ip, D, D, P(VP)]),
             "dg_mesh_periodic_jittered": (I, [I, I, D, D, D, D, D, C.c_uint64, P(VP)]),
             "dg_mesh_write_pmsh": (I, [VP, C.c_char_p]),
@@ -193,12 +195,16 @@ class Mesh:
         return cls(h)
 
     @classmethod
-    def read_gmsh(cls, path, genmesh_quirks=False):
+    def read_gmsh(cls, path, genmesh_quirks=False, periodic=False):
         """gmsh 2.2 .msh converted as the reference's genmesh.py does (with
-        genmesh_quirks, exactly, including its latent bugs)."""
+        genmesh_quirks, exactly, including its latent bugs).  periodic pairs
+        the open untagged sides on opposite edges of the bounding box."""
         h = C.c_void_p()
-        _check(lib().dg_mesh_read_gmsh(str(path).encode(), 1 if genmesh_quirks else 0, C.byref(h)))
-        return cls(h)
+        flags = (DG_GMSH_QUIRKS if genmesh_quirks else 0) | (DG_GMSH_PERIODIC if periodic else 0)
+        _check(lib().dg_mesh_read_gmsh(str(path).encode(), flags, C.byref(h)))
+        px, py = C.c_double(), C.c_double()
+        _check(lib().dg_mesh_period(h, C.byref(px), C.byref(py)))
+        return cls(h, (px.value, py.value))
 
     @classmethod
     def periodic_jittered(cls, nx, ny, x0, x1, y0, y1, jitter=0.1, seed=0):

@@ -79,11 +79,14 @@ int dg_mesh_read_pmsh(const char* path, dg_mesh** out) {
     GUARD_END
 }
 
-int dg_mesh_read_gmsh(const char* path, int genmesh_quirks, dg_mesh** out) {
+int dg_mesh_read_gmsh(const char* path, int flags, dg_mesh** out) {
     GUARD_BEGIN
     if (!path || !out) return fail(DG_ERR_ARG, "null argument");
+    if (flags & ~(DG_GMSH_QUIRKS | DG_GMSH_PERIODIC)) return fail(DG_ERR_ARG, "unknown gmsh flags");
+    if ((flags & DG_GMSH_QUIRKS) && (flags & DG_GMSH_PERIODIC))
+        return fail(DG_ERR_ARG, "periodic pairing is not part of genmesh.py's conversion (no quirk mode)");
     auto* h = new dg_mesh();
-    std::string why = read_gmsh(path, h->m, genmesh_quirks != 0);
+    std::string why = read_gmsh(path, h->m, (flags & DG_GMSH_QUIRKS) != 0, (flags & DG_GMSH_PERIODIC) != 0);
     if (!why.empty()) {
         delete h;
         return fail(why.rfind("cannot open", 0) == 0 ? DG_ERR_IO : DG_ERR_PARSE, why);
@@ -155,6 +158,13 @@ int dg_mesh_info(const dg_mesh* m, int* num_elem, int* num_sides, int* num_bound
         for (int r : m->m.right_elem) nb += (r < 0);
         *num_boundary_sides = nb;
     }
+    return DG_OK;
+}
+
+int dg_mesh_period(const dg_mesh* m, double* period_x, double* period_y) {
+    if (!m || !period_x || !period_y) return fail(DG_ERR_ARG, "null argument");
+    *period_x = m->m.period_x;
+    *period_y = m->m.period_y;
     return DG_OK;
 }
 

@@ -57,7 +57,86 @@ std::string read_pmsh(const char* path, Mesh& m) {
     return validate(m);
 }
 
-std::string read_gmsh(const char* path, Mesh& m, bool quirks) {
+// Periodic pairing of the open, untagged sides on the edges of the nodes'
+// bounding box: a side on x = xmin pairs with the side on x = xmax whose end
+// points are its own moved by the box width, and likewise in y.  The side met
+// first keeps its index and gains the partner's element as its right element
+// (the partner's face, traversed the other way as an interior side's right
+// face is); the partner side is removed and the rest renumbered in order.
+// genmesh.py has no periodic support; this is the pairing an interior side
+// would get if the box were wrapped.
+static std::string pair_periodic(const std::vector<double>& vx, const std::vector<double>& vy,
+                                 std::vector<std::pair<int, int>>& sidelist, std::vector<int>& left_elem,
+                                 std::vector<int>& right_elem, std::vector<int>& lsn, std::vector<int>& rsn,
+                                 std::vector<char>& has_right, std::vector<std::array<int, 3>>& es, int& ns,
+                                 double& period_x, double& period_y, double& x0, double& y0) {
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+    for (int s = 0; s < ns; s++)
+        for (int v : {sidelist[s].first, sidelist[s].second}) {
+            xmin = std::min(xmin, vx[v]); xmax = std::max(xmax, vx[v]);
+            ymin = std::min(ymin, vy[v]); ymax = std::max(ymax, vy[v]);
+        }
+    const double tol = 1e-9 * std::max(xmax - xmin, ymax - ymin);
+    // open untagged sides by box edge: 0 x = xmin, 1 x = xmax, 2 y = ymin, 3 y = ymax
+    std::vector<int> on[4];
+    for (int s = 0; s < ns; s++) {
+        if (has_right[s] || right_elem[s] < 0) continue;
+        const int a = sidelist[s].first, b = sidelist[s].second;
+        auto at = [&](const std::vector<double>& c, double v) { return std::fabs(c[a] - v) <= tol && std::fabs(c[b] - v) <= tol; };
+        const int edge = at(vx, xmin) ? 0 : at(vx, xmax) ? 1 : at(vy, ymin) ? 2 : at(vy, ymax) ? 3 : -1;
+        if (edge < 0) return "side " + std::to_string(s) + " has one element, no boundary tag and is not on the box";
+        on[edge].push_back(s);
+    }
+    std::vector<char> gone(ns, 0);
+    for (int axis = 0; axis < 2; axis++) {
+        auto& lo = on[2 * axis];
+        auto& hi = on[2 * axis + 1];
+        if (lo.size() != hi.size())
+            return std::string("periodic pairing: ") + std::to_string(lo.size()) + " open sides on " +
+                   (axis ? "y = ymin" : "x = xmin") + " against " + std::to_string(hi.size()) + " on the opposite edge";
+        if (lo.empty()) continue;
+        // the coordinate along the edge: y for the x edges, x for the y edges
+        const std::vector<double>& t = axis ? vx : vy;
+        auto lo_end = [&](int s) { return std::min(t[sidelist[s].first], t[sidelist[s].second]); };
+        auto hi_end = [&](int s) { return std::max(t[sidelist[s].first], t[sidelist[s].second]); };
+        auto by_lo = [&](int a, int b) { return lo_end(a) < lo_end(b); };
+        std::sort(lo.begin(), lo.end(), by_lo);
+        std::sort(hi.begin(), hi.end(), by_lo);
+        for (size_t i = 0; i < lo.size(); i++) {
+            const int a = lo[i], b = hi[i];
+            if (std::fabs(lo_end(a) - lo_end(b)) > tol || std::fabs(hi_end(a) - hi_end(b)) > tol)
+                return "periodic pairing: side " + std::to_string(a) + " has no partner on the opposite edge";
+            const int keep = std::min(a, b), drop = std::max(a, b);
+            right_elem[keep] = left_elem[drop];
+            rsn[keep] = lsn[drop];
+            has_right[keep] = 1;
+            es[left_elem[drop]][lsn[drop]] = keep;
+            gone[drop] = 1;
+        }
+        (axis ? period_y : period_x) = axis ? ymax - ymin : xmax - xmin;
+    }
+    x0 = xmin;
+    y0 = ymin;
+    std::vector<int> renum(ns, -1);
+    int n = 0;
+    for (int s = 0; s < ns; s++) {
+        if (gone[s]) continue;
+        renum[s] = n;
+        sidelist[n] = sidelist[s];
+        left_elem[n] = left_elem[s];
+        right_elem[n] = right_elem[s];
+        lsn[n] = lsn[s];
+        rsn[n] = rsn[s];
+        has_right[n] = has_right[s];
+        n++;
+    }
+    for (auto& e : es)
+        for (int& s : e) s = renum[s];
+    ns = n;
+    return std::string();
+}
+
+std::string read_gmsh(const char* path, Mesh& m, bool quirks, bool periodic) {
     FILE* f = fopen(path, "r");
     if (!f) return std::string("cannot open mesh file ") + path;
     char line[4096];
@@ -155,6 +234,13 @@ std::string read_gmsh(const char* path, Mesh& m, bool quirks) {
             ns++;
         }
     }
+    double period_x = 0, period_y = 0, x0 = 0, y0 = 0;
+    if (periodic) {
+        if (quirks) return "periodic pairing is not part of genmesh.py's conversion";
+        std::string why = pair_periodic(vx, vy, sidelist, left_elem, right_elem, lsn, rsn, has_right, es, ns, period_x,
+                                        period_y, x0, y0);
+        if (!why.empty()) return why;
+    }
     if (!quirks)
         for (int s = 0; s < ns; s++)
             if (!has_right[s] && right_elem[s] >= 0)
@@ -184,6 +270,10 @@ std::string read_gmsh(const char* path, Mesh& m, bool quirks) {
     m = Mesh{};
     m.K = K;
     m.S = ns;
+    m.period_x = period_x;
+    m.period_y = period_y;
+    m.x0 = x0;
+    m.y0 = y0;
     for (int i = 0; i < K; i++) {
         const auto& e = tri[i];
         m.V1x.push_back(vx[e[0]]); m.V1y.push_back(vy[e[0]]);

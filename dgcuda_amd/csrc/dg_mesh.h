@@ -43,7 +43,12 @@ std::string read_pmsh(const char* path, Mesh& m);
 // index (element K + code).  Without it the element's own vertex order is
 // swapped, the sort leaves boundary sides' missing right elements alone,
 // and a side that belongs to one element but carries no tag is an error.
-std::string read_gmsh(const char* path, Mesh& m, bool genmesh_quirks);
+// With `periodic` (clean mode only) the open untagged sides on the edges of
+// the nodes' bounding box are first paired with their translates on the
+// opposite edge (the box's width / height becomes the mesh's period), the
+// partner side removed and the rest renumbered in order; tagged sides stay
+// boundary sides, so a channel periodic in x with walls in y reads too.
+std::string read_gmsh(const char* path, Mesh& m, bool genmesh_quirks, bool periodic);
 std::string write_pmsh(const char* path, const Mesh& m);
 
 // n_x * n_y squares on [x0,x1] x [y0,y1], periodic in both directions, each

@@ -124,13 +124,23 @@ int dg_mesh_read_pmsh(const char* path, dg_mesh** out);
  * builds a .pmsh: sides numbered as first met over (element, face 0..2), the
  * first element meeting a side is its left element, line elements tagged
  * 10000 / 20000 / 30000 make reflecting / outflow / inflow sides, boundary
- * sides sorted to the front.  genmesh_quirks = 1 reproduces the converter
+ * sides sorted to the front.  flags = DG_GMSH_QUIRKS reproduces the converter
  * exactly, including its two latent bugs (a negative-Jacobian element swaps
  * shared vertex coordinates; the front-sort writes through a negative
  * element index); 0 swaps the element's own vertices instead, leaves
  * boundary sides' missing right elements alone and rejects untagged open
- * sides. */
-int dg_mesh_read_gmsh(const char* path, int genmesh_quirks, dg_mesh** out);
+ * sides.  DG_GMSH_PERIODIC (not with DG_GMSH_QUIRKS: genmesh.py has no
+ * periodic support) first pairs the open untagged sides on the edges of the
+ * nodes' bounding box with their translates on the opposite edge, which sets
+ * the mesh's period (dg_mesh_period); the partner side is removed and the
+ * others keep their order.  Unknown flags: DG_ERR_ARG; an open side with
+ * no tag and no partner: DG_ERR_PARSE. */
+enum { DG_GMSH_QUIRKS = 1, DG_GMSH_PERIODIC = 2 };
+int dg_mesh_read_gmsh(const char* path, int flags, dg_mesh** out);
+/* the mesh's periodic extent in x and y (0 for a direction that does not
+ * wrap); the initial conditions measure distances to the nearest image over
+ * it.  No counterpart in the reference, whose meshes are not periodic. */
+int dg_mesh_period(const dg_mesh* m, double* period_x, double* period_y);
 /* builds a mesh from the arrays read_mesh returns, in its order:
  * V[6K] = V1x|V1y|V2x|V2y|V3x|V3y, elem_s[3K] = elem_s1|elem_s2|elem_s3,
  * sides[4S] = x1|y1|x2|y2, side_int[4S] = left_elem|right_elem|left_side_number|right_side_number.

@@ -73,3 +73,53 @@ def relabelled(m):
     inner = si[1] >= 0
     si[3][inner] = (si[3][inner] + 2) % 3
     return dg.Mesh.from_arrays(Vn, esn, sd, si, m.period)
+
+
+def write_msh(path, nodes, lines, tris):
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(nodes))
+        for i, (x, y) in enumerate(nodes):
+            f.write("%d %r %r 0\n" % (i + 1, x, y))
+        f.write("$EndNodes\n$Elements\n%d\n" % (len(lines) + len(tris)))
+        k = 1
+        for tag, a, b in lines:
+            f.write("%d 1 2 %d 1 %d %d\n" % (k, tag, a + 1, b + 1))
+            k += 1
+        for a, b, c in tris:
+            f.write("%d 2 2 100 6 %d %d %d\n" % (k, a + 1, b + 1, c + 1))
+            k += 1
+        f.write("$EndElements\n")
+
+
+def write_elements_msh(path, V, lines=()):
+    """The elements of V[6, K] as gmsh triangles in order, vertex order kept,
+    nodes shared by exact coordinates; lines are (tag, (x, y), (x, y))."""
+    ids, nodes = {}, []
+
+    def node(x, y):
+        x, y = float(x), float(y)
+        if (x, y) not in ids:
+            ids[(x, y)] = len(nodes)
+            nodes.append((x, y))
+        return ids[(x, y)]
+
+    tris = [tuple(node(V[2 * c, e], V[2 * c + 1, e]) for c in range(3)) for e in range(V.shape[1])]
+    segs = [(tag, node(*a), node(*b)) for tag, a, b in lines]
+    write_msh(path, nodes, segs, tris)
+
+
+def generated(nx=6, ny=5):
+    return dg.Mesh.periodic_jittered(nx, ny, 0.0, 3.0, 0.0, 2.0, jitter=0.1, seed=7)
+
+
+def channel_lines(V, ylo, yhi):
+    """Inflow (30000) lines on the element faces on y = ylo, outflow (20000)
+    on y = yhi."""
+    out = []
+    for e in range(V.shape[1]):
+        pts = [(V[2 * c, e], V[2 * c + 1, e]) for c in range(3)]
+        for k in range(3):
+            a, b = pts[k], pts[(k + 1) % 3]
+            if a[1] == b[1] and a[1] in (ylo, yhi):
+                out.append((30000 if a[1] == ylo else 20000, a, b))
+    return out

@@ -13,7 +13,17 @@ Pins:
 * an open side without a boundary tag is an error in the clean mode and
   gets the converter's default right element 0 in the quirk mode;
 * the gmsh 1.0 file among the reference's meshes (cyl16.msh, whose .pmsh
-  the reference left empty) is refused with a parse error.
+  the reference left empty) is refused with a parse error;
+* periodic pairing (not part of genmesh.py): the elements of a generated
+  periodic mesh written as a .msh without its wrap and read back paired
+  give the generated mesh's period, side count 3K/2 and, for seeded states,
+  the oracle's RHS of the generated mesh (a side paired with the wrong
+  partner changes it at O(1)); a channel periodic in x with tagged outflow
+  (top) and inflow (bottom) sides keeps them as boundary sides in front,
+  outflow first, and preserves a constant state (the inflow / outflow ghost
+  is the problem's own state; the reference's reflecting ghost uses the
+  radial normal of the supersonic vortex's arcs, so a flat wall is not a
+  test of it); unmatched edges and the quirk mode are refused.
 """
 import os
 
@@ -21,7 +31,8 @@ import numpy as np
 import pytest
 
 import dgcuda_amd as dg
-from helpers import GOLD_DIR
+import oracle as orc
+from helpers import GOLD_DIR, channel_lines, generated, perturbed_state, rel_max, write_elements_msh, write_msh
 
 GM = os.path.join(GOLD_DIR, "gmsh")
 
@@ -42,22 +53,6 @@ def lf_text(m):
 def test_shipped_pairs_convert_exactly(name, quirks):
     m = dg.Mesh.read_gmsh(os.path.join(GM, name + ".msh"), genmesh_quirks=quirks)
     assert lf_text(m) == open(os.path.join(GM, name + ".pmsh")).read()
-
-
-def write_msh(path, nodes, lines, tris):
-    with open(path, "w") as f:
-        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(nodes))
-        for i, (x, y) in enumerate(nodes):
-            f.write("%d %r %r 0\n" % (i + 1, x, y))
-        f.write("$EndNodes\n$Elements\n%d\n" % (len(lines) + len(tris)))
-        k = 1
-        for tag, a, b in lines:
-            f.write("%d 1 2 %d 1 %d %d\n" % (k, tag, a + 1, b + 1))
-            k += 1
-        for a, b, c in tris:
-            f.write("%d 2 2 100 6 %d %d %d\n" % (k, a + 1, b + 1, c + 1))
-            k += 1
-        f.write("$EndElements\n")
 
 
 # unit square split along a diagonal; the second triangle is clockwise
@@ -102,3 +97,75 @@ def test_gmsh_1_format_refused(tmp_path):
     with pytest.raises(dg.DGError) as e:
         dg.Mesh.read_gmsh(path)
     assert e.value.code == dg.STATUS_CODES["DG_ERR_PARSE"]
+
+
+@pytest.mark.parametrize("p", [1, 3])
+@pytest.mark.parametrize("physics", ["advection", "euler"])
+def test_periodic_pairing_matches_generated_mesh(tmp_path, physics, p):
+    g = generated()
+    V = g.arrays()[0]
+    path = tmp_path / "box.msh"
+    write_elements_msh(path, V)
+    m = dg.Mesh.read_gmsh(path, periodic=True)
+    assert m.period == (3.0, 2.0) == g.period
+    assert m.num_elem == g.num_elem and m.num_sides == 3 * m.num_elem // 2 and m.num_boundary_sides == 0
+    assert (m.arrays()[0] == V).all()
+    if physics == "advection":
+        prob = dg.advection_gaussian(a=(1.0, 0.5), center=(1.5, 1.0), k=4.0)
+    else:
+        prob = dg.euler_vortex(vortices=((1.5, 1.0, 2.0),))
+    og = orc.Oracle.from_mesh(g, p, prob)
+    om = orc.Oracle.from_mesh(m, p, prob)
+    u = perturbed_state(og, prob, g.period, seed=p, amp=1e-2)
+    assert rel_max(om.project_ic(orc.ic_array(prob, m.period)), og.project_ic(orc.ic_array(prob, g.period))) == 0
+    err = rel_max(om.rhs(u, dt=0.01), og.rhs(u, dt=0.01))
+    print("periodic pairing %s p=%d rel max %.3e" % (physics, p, err))
+    assert err <= 1e-13
+    # planted: two wrapped sides exchange partners; the RHS moves at O(1)
+    V2, es2, sd2, si2 = m.arrays()
+    wrap = np.nonzero((sd2[0] == sd2[2]) & np.isin(sd2[0], (0.0, 3.0)))[0]
+    a, b = wrap[0], wrap[len(wrap) // 2]
+    ra, fa, rb, fb = si2[1][a], si2[3][a], si2[1][b], si2[3][b]
+    si2[1][a], si2[3][a], si2[1][b], si2[3][b] = rb, fb, ra, fa
+    es2[fa][ra], es2[fb][rb] = b, a
+    bad = dg.Mesh.from_arrays(V2, es2, sd2, si2, m.period)
+    assert rel_max(orc.Oracle.from_mesh(bad, p, prob).rhs(u, dt=0.01), og.rhs(u, dt=0.01)) > 1e-3
+
+
+def test_periodic_channel(tmp_path):
+    nx, ny = 6, 5
+    V = generated(nx, ny).arrays()[0]
+    path = tmp_path / "channel.msh"
+    write_elements_msh(path, V, channel_lines(V, 0.0, 2.0))
+    m = dg.Mesh.read_gmsh(path, periodic=True)
+    assert m.period == (3.0, 0.0)
+    assert m.num_boundary_sides == 2 * nx and m.num_sides == (3 * m.num_elem + 2 * nx) // 2
+    si = m.arrays()[3]
+    assert (si[1][:nx] == -2).all() and (si[1][nx:2 * nx] == -3).all() and (si[1][2 * nx:] >= 0).all()
+    prob = dg.euler_constant((1.0, 0.5, 0.3, 2.5))
+    o = orc.Oracle.from_mesh(m, 2, prob)
+    u = o.project_ic(orc.ic_array(prob, m.period))
+    # rounding level: |U| ~ 2.5 times M^-1 ~ 1/J ~ 30 on these elements
+    assert np.abs(o.rhs(u, dt=1.0)).max() <= 1e-12
+
+
+def test_periodic_pairing_refusals(tmp_path):
+    V = generated().arrays()[0].copy()
+    path = tmp_path / "box.msh"
+    write_elements_msh(path, V)
+    with pytest.raises(dg.DGError) as e:
+        dg.Mesh.read_gmsh(path, genmesh_quirks=True, periodic=True)
+    assert e.value.code == dg.STATUS_CODES["DG_ERR_ARG"]
+    import ctypes
+    h = ctypes.c_void_p()
+    assert dg.lib().dg_mesh_read_gmsh(str(path).encode(), 4, ctypes.byref(h)) == dg.STATUS_CODES["DG_ERR_ARG"]
+    # one node of the right edge moved along it: its two sides lose their partners
+    right = V[0::2] == 3.0
+    ys = np.unique(V[1::2][right])
+    y = ys[len(ys) // 2]
+    V[1::2][right & (V[1::2] == y)] = y + 0.01
+    write_elements_msh(path, V)
+    with pytest.raises(dg.DGError) as e:
+        dg.Mesh.read_gmsh(path, periodic=True)
+    assert e.value.code == dg.STATUS_CODES["DG_ERR_PARSE"]
+    assert "partner" in str(e.value)
