@@ -13,7 +13,9 @@
 //   gather + M^-1      (time_integrator.cu:130-188, M = J*I)
 //   RK stage update    (time_integrator.cu:194-218, 445-464)
 //   CFL wave speed     (euler.cu:252-278) of the new state on the last stage.
-// The FP64 instantiation follows the reference's operation order.
+// Euler up to p = 3 uses the normal-flux forms of the matrix-core kernels
+// (fewer operations and registers); otherwise the reference's operation
+// order is kept.
 #pragma once
 #include "dg_kernels.h"
 
@@ -54,9 +56,9 @@ __device__ __forceinline__ void boundary_ghost(const double* bnd, int w, int j, 
 template <int N, int P, typename R>
 __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int kind) {
     constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1;
-    // FP32 Euler up to p = 3 uses the normal-flux forms of the matrix-core
-    // kernels (fewer operations; p >= 4 keeps its register allocation)
-    constexpr bool NF_FORM = sizeof(R) == 4 && N == 4 && P <= 3;
+    // Euler up to p = 3 uses the normal-flux forms of the matrix-core
+    // kernels (fewer operations and registers; p >= 4 keeps its allocation)
+    constexpr bool NF_FORM = N == 4 && P <= 3;
     const TabP<R, P>& T = tab<R, P>();
     const bool valid = e < a.K;
     const int ee = valid ? e : a.K - 1;
@@ -88,7 +90,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
                 for (int v = 0; v < N; v++) U[v] += u[v][m] * T.B[m][q];
             R fx[N], fy[N];
             if constexpr (NF_FORM) {
-                // FP32 Euler: fluxes contracted with the inverse map first
+                // Euler: fluxes contracted with the inverse map first
                 // (metric_flux), then with the weighted basis gradients
                 metric_flux<R>(a.phys, U, xr, yr, xs, ys, fx, fy);
 #pragma unroll
@@ -153,7 +155,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
             if constexpr (P <= VALU_BOUNDARY_MAX_P)
                 if (bnd) boundary_ghost<N, NQ1, R>(a.bnd, w, j, Uo, Un);
             if constexpr (NF_FORM) {
-                // FP32 Euler: normal fluxes of both sides, the side's sign in
+                // Euler: normal fluxes of both sides, the side's sign in
                 // the speed (the matrix-core kernel's form; both elements of
                 // the side still get bit-identical values)
                 R Fo[N], Fb[N], so, sb;
