@@ -114,7 +114,7 @@ hipError_t launch_stage_mfma(int N, int p, const StageArgs<R>& a, const MfmaArgs
 bool mfma_supported(int N, int p);
 // dynamic LDS of a launch, `rs` = bytes per value of the bulk precision
 size_t mfma_lds_bytes(int N, int hcap, int rs);
-int mfma_resident_groups(int N, int p, int hcap, int rs);
+int mfma_resident_groups(int N, int p, int hcap, int rs, bool bnd);
 hipError_t mfma_layout_probe(float* host_out);
 hipError_t mfma_layout_probe_f64(double* host_out);
 constexpr int VALU_TILE_ELEMS = 128;
@@ -130,5 +130,35 @@ hipError_t launch_project_ic(int p, R* c, const StateIx& ix, const double* V, in
 template <typename R>
 hipError_t launch_audit(const R* c, const StateIx& ix, const double* Jd, int K, double b0, double gamma,
                         double* part, int nblocks, hipStream_t st);
+
+// Ghost state UR of point j of a boundary side from the left trace UL
+// (eval_left_right, conserv_kernels.cu:647-671): reflecting keeps rho and E
+// and the velocity's component along the wall (supersonic.cu:64-108, wall
+// normal precomputed per point); inflow and outflow take the problem's
+// state at the point (euler.cu:177-241, precomputed in FP64).
+template <int N, int NQ1, typename R>
+__device__ __forceinline__ void boundary_ghost(const double* bnd, int w, int j, const R* UL, R* UR) {
+    // the point's four values go to scalars first: indexing the pointer in a
+    // loop made the compiler keep the ghost in a private (scratch) array
+    const double* g = bnd + ((size_t)(w & NB_ELEM_MASK) * NQ1 + j) * 4;
+    const double g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
+    if (((w >> 28) & 3) == BC_REFLECTING) {
+        UR[0] = UL[0];
+        if constexpr (N == 4) {
+            const R Nx = (R)g0, Ny = (R)g1;
+            const R t = UL[1] * Ny - UL[2] * Nx;
+            UR[1] = t * Ny;
+            UR[2] = -t * Nx;
+            UR[3] = UL[3];
+        }
+    } else {
+        UR[0] = (R)g0;
+        if constexpr (N == 4) {
+            UR[1] = (R)g1;
+            UR[2] = (R)g2;
+            UR[3] = (R)g3;
+        }
+    }
+}
 
 }  // namespace dgmi

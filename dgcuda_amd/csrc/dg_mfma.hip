@@ -71,7 +71,8 @@ hipError_t mfma_layout_probe_f64(double* host_out) {
 template <typename R>
 constexpr int mfma_minw() { return sizeof(R) == 4 ? DGMI_MFMA_MINW : 2; }
 
-template <int N, int P, int KIND, typename R>
+// BND: the variant for meshes with boundary sides (Euler only)
+template <int N, int P, int KIND, typename R, bool BND = false>
 __global__ void __launch_bounds__(256, mfma_minw<R>()) k_stage_mfma(StageArgs<R> a, MfmaArgs mx) {
     using V = typename MfmaT<R>::V;
     constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(256, mfma_minw<R>()) k_stage_mfma(StageArgs<R>
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int tc = t; tc < tend; tc += stride)
-        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND, R>(a, mx, tabA, st, hid, gc, dt, tc, walk(tc + stride),
+        lam = fmax(lam, mfma_process_tile_pipe<N, P, KIND, R, BND>(a, mx, tabA, st, hid, gc, dt, tc, walk(tc + stride),
                                                               walk(tc + 2 * stride), fd, ts));
 
     if constexpr (LAST) {
@@ -168,17 +169,25 @@ template hipError_t launch_mfma_pack<double>(const double*, const double*, const
                                              hipStream_t);
 
 // workgroups of the matrix-core kernel that fit on the device at once
+template <typename R, bool BND>
+static const void* stage2_kernel(int N, int p) {
+    if constexpr (BND) {
+        if (N != 4) return nullptr;
+    } else {
+        if (N == 1 && p == 4) return reinterpret_cast<const void*>(&k_stage_mfma<1, 4, ST_RK4_2, R>);
+        if (N == 1 && p == 3) return reinterpret_cast<const void*>(&k_stage_mfma<1, 3, ST_RK4_2, R>);
+        if (N == 1 && p == 2) return reinterpret_cast<const void*>(&k_stage_mfma<1, 2, ST_RK4_2, R>);
+        if (N == 1 && p == 1) return reinterpret_cast<const void*>(&k_stage_mfma<1, 1, ST_RK4_2, R>);
+    }
+    if (N == 4 && p == 4) return reinterpret_cast<const void*>(&k_stage_mfma<4, 4, ST_RK4_2, R, BND>);
+    if (N == 4 && p == 3) return reinterpret_cast<const void*>(&k_stage_mfma<4, 3, ST_RK4_2, R, BND>);
+    if (N == 4 && p == 2) return reinterpret_cast<const void*>(&k_stage_mfma<4, 2, ST_RK4_2, R, BND>);
+    if (N == 4 && p == 1) return reinterpret_cast<const void*>(&k_stage_mfma<4, 1, ST_RK4_2, R, BND>);
+    return nullptr;
+}
 template <typename R>
-static int resident_groups(int N, int p, int hcap) {
-    const void* fn = nullptr;
-    if (N == 4 && p == 4) fn = reinterpret_cast<const void*>(&k_stage_mfma<4, 4, ST_RK4_2, R>);
-    if (N == 4 && p == 3) fn = reinterpret_cast<const void*>(&k_stage_mfma<4, 3, ST_RK4_2, R>);
-    if (N == 1 && p == 4) fn = reinterpret_cast<const void*>(&k_stage_mfma<1, 4, ST_RK4_2, R>);
-    if (N == 1 && p == 3) fn = reinterpret_cast<const void*>(&k_stage_mfma<1, 3, ST_RK4_2, R>);
-    if (N == 4 && p == 2) fn = reinterpret_cast<const void*>(&k_stage_mfma<4, 2, ST_RK4_2, R>);
-    if (N == 4 && p == 1) fn = reinterpret_cast<const void*>(&k_stage_mfma<4, 1, ST_RK4_2, R>);
-    if (N == 1 && p == 2) fn = reinterpret_cast<const void*>(&k_stage_mfma<1, 2, ST_RK4_2, R>);
-    if (N == 1 && p == 1) fn = reinterpret_cast<const void*>(&k_stage_mfma<1, 1, ST_RK4_2, R>);
+static int resident_groups(int N, int p, int hcap, bool bnd) {
+    const void* fn = bnd ? stage2_kernel<R, true>(N, p) : stage2_kernel<R, false>(N, p);
     if (!fn) return 0;
     int per_cu = 0, dev = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, mfma_lds_bytes(N, hcap, (int)sizeof(R))) !=
@@ -188,8 +197,8 @@ static int resident_groups(int N, int p, int hcap) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     return per_cu * cus;
 }
-int mfma_resident_groups(int N, int p, int hcap, int rs) {
-    return rs == 8 ? resident_groups<double>(N, p, hcap) : resident_groups<float>(N, p, hcap);
+int mfma_resident_groups(int N, int p, int hcap, int rs, bool bnd) {
+    return rs == 8 ? resident_groups<double>(N, p, hcap, bnd) : resident_groups<float>(N, p, hcap, bnd);
 }
 
 // FP64 launches take up to 86 KB of dynamic LDS: raise each kernel's
@@ -203,45 +212,54 @@ static hipError_t allow_large_lds(K* fn) {
     return e;
 }
 
-template <int N, int P, typename R>
+template <int N, int P, typename R, bool BND>
 static hipError_t prepare_lds() {
     if constexpr (sizeof(R) == 4) {
         return hipSuccess;
     } else {
         static const hipError_t once = [] {
-            hipError_t e = allow_large_lds(&k_stage_mfma<N, P, ST_RHS, R>);
-            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK4_1, R>);
-            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK4_2, R>);
-            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK4_3, R>);
-            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK4_4, R>);
-            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK2_1, R>);
-            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK2_2, R>);
+            hipError_t e = allow_large_lds(&k_stage_mfma<N, P, ST_RHS, R, BND>);
+            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK4_1, R, BND>);
+            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK4_2, R, BND>);
+            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK4_3, R, BND>);
+            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK4_4, R, BND>);
+            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK2_1, R, BND>);
+            if (e == hipSuccess) e = allow_large_lds(&k_stage_mfma<N, P, ST_RK2_2, R, BND>);
             return e;
         }();
         return once;
     }
 }
 
-template <int N, int P, typename R>
+template <int N, int P, typename R, bool BND = false>
 static hipError_t launch_np(const StageArgs<R>& a, const MfmaArgs& mx, dim3 grid, hipStream_t st) {
     const size_t lds = mfma_lds_bytes(N, mx.hcap, (int)sizeof(R));
-    const hipError_t pe = prepare_lds<N, P, R>();
+    const hipError_t pe = prepare_lds<N, P, R, BND>();
     if (pe != hipSuccess) return pe;
     switch (a.kind) {
-        case ST_RHS: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RHS, R>), grid, dim3(256), lds, st, a, mx); break;
-        case ST_RK4_1: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK4_1, R>), grid, dim3(256), lds, st, a, mx); break;
-        case ST_RK4_2: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK4_2, R>), grid, dim3(256), lds, st, a, mx); break;
-        case ST_RK4_3: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK4_3, R>), grid, dim3(256), lds, st, a, mx); break;
-        case ST_RK4_4: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK4_4, R>), grid, dim3(256), lds, st, a, mx); break;
-        case ST_RK2_1: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK2_1, R>), grid, dim3(256), lds, st, a, mx); break;
-        case ST_RK2_2: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK2_2, R>), grid, dim3(256), lds, st, a, mx); break;
+        case ST_RHS: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RHS, R, BND>), grid, dim3(256), lds, st, a, mx); break;
+        case ST_RK4_1: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK4_1, R, BND>), grid, dim3(256), lds, st, a, mx); break;
+        case ST_RK4_2: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK4_2, R, BND>), grid, dim3(256), lds, st, a, mx); break;
+        case ST_RK4_3: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK4_3, R, BND>), grid, dim3(256), lds, st, a, mx); break;
+        case ST_RK4_4: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK4_4, R, BND>), grid, dim3(256), lds, st, a, mx); break;
+        case ST_RK2_1: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK2_1, R, BND>), grid, dim3(256), lds, st, a, mx); break;
+        case ST_RK2_2: hipLaunchKernelGGL((k_stage_mfma<N, P, ST_RK2_2, R, BND>), grid, dim3(256), lds, st, a, mx); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+// a.bnd set: the mesh has boundary sides and the launch takes the BND
+// variant (Euler only; the host refuses advection on such meshes here)
 template <typename R>
 hipError_t launch_stage_mfma(int N, int p, const StageArgs<R>& a, const MfmaArgs& mx, dim3 grid, hipStream_t st) {
+    if (a.bnd) {
+        if (N == 4 && p == 4) return launch_np<4, 4, R, true>(a, mx, grid, st);
+        if (N == 4 && p == 3) return launch_np<4, 3, R, true>(a, mx, grid, st);
+        if (N == 4 && p == 2) return launch_np<4, 2, R, true>(a, mx, grid, st);
+        if (N == 4 && p == 1) return launch_np<4, 1, R, true>(a, mx, grid, st);
+        return hipErrorInvalidValue;
+    }
     if (N == 4 && p == 4) return launch_np<4, 4, R>(a, mx, grid, st);
     if (N == 4 && p == 3) return launch_np<4, 3, R>(a, mx, grid, st);
     if (N == 4 && p == 2) return launch_np<4, 2, R>(a, mx, grid, st);

@@ -313,7 +313,7 @@ __device__ __forceinline__ double inv_jacobian(double J) { return 1.0 / J; }
 // tiles of this launch (the caller repeats the last tile where the walk
 // ends), so no load sits under a branch.  The A operands are read from the
 // LDS table `tabA`.
-template <int N, int P, int KIND, typename R>
+template <int N, int P, int KIND, typename R, bool BND = false>
 __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, const MfmaArgs& mx,
                                                          const typename MfmaT<R>::V* tabA, MfmaTileState<N, P, R>& st,
                                                          int (&hid)[MfmaTileState<N, P, R>::HPW], TileGeo<R>& gc,
@@ -457,6 +457,11 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
             const int wd = face_word(r[3]);
             const int slot = wd & 0xFFF, fn = (wd >> 12) & 3;
             const bool left = (wd >> 14) & 1;
+            // boundary side (the element is its left element, the slot its
+            // own): the ghost state of the point replaces the neighbour trace.
+            // Only the BND instantiation carries this code, so the kernels of
+            // meshes without boundary sides keep their register allocation.
+            const bool bside = BND && ((wd >> 15) & 1);
             const R* own = ts + el * TS + pt * N;
             const R* nbt = ts + slot * TS + (fn * NQ1 + NQ1 - 1 - jo) * N;
             if constexpr (N == 4) {
@@ -471,6 +476,8 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
                     uo[v] = own[v];
                     un[v] = nbt[v];
                 }
+                if constexpr (BND)
+                    if (bside) boundary_ghost<N, NQ1, R>(a.bnd, gp(a.nbr)[f * a.Kp + ee], jo, uo, un);
                 normal_flux_speed<R>(a.phys, uo, r[0], r[1], Fo, so);
                 normal_flux_speed<R>(a.phys, un, r[0], r[1], Fb, sb);
                 const R lm = fmax(so, sb);
@@ -486,6 +493,8 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
                     UL[v] = left ? uo : un;
                     UR[v] = left ? un : uo;
                 }
+                if constexpr (BND)
+                    if (bside) boundary_ghost<N, NQ1, R>(a.bnd, gp(a.nbr)[f * a.Kp + ee], jo, UL, UR);
                 R fxl[N], fyl[N], fxr[N], fyr[N];
                 flux<N, R>(a.phys, UL, fxl, fyl);
                 flux<N, R>(a.phys, UR, fxr, fyr);

@@ -247,8 +247,8 @@ static std::mutex& tune_mu() {
     static std::mutex m;
     return m;
 }
-static std::map<std::tuple<int, int, int, int>, int>& tune_cache() {
-    static std::map<std::tuple<int, int, int, int>, int> c;
+static std::map<std::tuple<int, int, int, int, int>, int>& tune_cache() {
+    static std::map<std::tuple<int, int, int, int, int>, int> c;
     return c;
 }
 
@@ -337,8 +337,10 @@ class SolverT : public Solver {
         std::string why = elem_geometry(m, perm, pr.quirks, g);
         if (!why.empty()) return Status::err(DG_ERR_UNSUPPORTED, why);
         has_boundary = !g.bfaces.empty();
-        if (has_boundary && p > VALU_BOUNDARY_MAX_P)
-            return Status::err(DG_ERR_UNSUPPORTED, "boundary sides are supported up to p = 3");
+        if (has_boundary && p > VALU_BOUNDARY_MAX_P && !(N == 4 && mfma_supported(N, p)))
+            return Status::err(DG_ERR_UNSUPPORTED,
+                               "boundary sides are supported up to p = 3 (lane kernel) and for Euler up to p = 4 "
+                               "(matrix-core kernels)");
         min_r = g.min_r;
         Status st = ensure_device_ready();
         if (!st) return st;
@@ -416,6 +418,13 @@ class SolverT : public Solver {
             for (int e = b; e < end; e++)
                 for (int f = 0; f < 3; f++) {
                     const int wv = h_nbr[f][e];
+                    if (wv < 0) {
+                        // boundary side: own slot and face, left, flag bit 15
+                        // (the kernel reads the ghost data through the
+                        // neighbour word)
+                        words[f * (size_t)Kp + e] = (uint16_t)((e - b) | (f << 12) | (1 << 14) | (1 << 15));
+                        continue;
+                    }
                     const int nb = wv & NB_ELEM_MASK, fn = (wv >> 28) & 3, left = (wv >> 30) & 1;
                     int slot;
                     if (nb >= b && nb < end) {
@@ -471,8 +480,10 @@ class SolverT : public Solver {
         const bool mf = mfma_supported(N, p);
         if ((kind == 2 || kind == 3) && !mf)
             return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs p = 1..4");
-        if ((kind == 2 || kind == 3) && has_boundary)
-            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs a mesh without boundary sides");
+        if ((kind == 2 || kind == 3) && has_boundary && N != 4)
+            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel with boundary sides needs Euler");
+        if (kind == 1 && has_boundary && p > VALU_BOUNDARY_MAX_P)
+            return Status::err(DG_ERR_UNSUPPORTED, "lane kernel with boundary sides needs p <= 3");
         if (kind < 1 || kind > 3) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1, 2 or 3");
         s = LaunchShape{};
         s.kind = kind;
@@ -485,7 +496,7 @@ class SolverT : public Solver {
             if (!st) return st;
             if (kind == 2) {
                 // persistent: as many workgroups as fit on the device at once
-                const int res = mfma_resident_groups(N, p, mx.hcap, (int)sizeof(R));
+                const int res = mfma_resident_groups(N, p, mx.hcap, (int)sizeof(R), has_boundary);
                 if (res >= 8) s.grid = std::min(s.grid, res / 8 * 8);
             }
         }
@@ -502,7 +513,7 @@ class SolverT : public Solver {
         if (!mfma_supported(N, p)) return Status::ok();
         int lk = 0;
         while (lk < 30 && (1 << (lk + 1)) <= K) lk++;
-        const auto key = std::make_tuple(N, p, (int)sizeof(R), lk);
+        const auto key = std::make_tuple(N, p, (int)sizeof(R), lk, (int)has_boundary);
         {
             std::lock_guard<std::mutex> lk_(tune_mu());
             auto it = tune_cache().find(key);
@@ -518,7 +529,7 @@ class SolverT : public Solver {
         HIPST(hipEventCreate(&e0));
         HIPST(hipEventCreate(&e1));
         float best = 0.f;
-        int best_kind = 1;
+        int best_kind = 0;
         const bool log = getenv("DGMI_TUNE_LOG") != nullptr;
         Status st;
         for (int kind : {1, 2, 3}) {
@@ -536,7 +547,7 @@ class SolverT : public Solver {
                 break;
             }
             if (log) fprintf(stderr, "dgmi autotune N=%d p=%d K=%d kind %d: %.4f ms per launch\n", N, p, K, kind, ms / 3);
-            if (kind == 1 || ms < best) {
+            if (best_kind == 0 || ms < best) {
                 best = ms;
                 best_kind = kind;
             }
@@ -544,6 +555,7 @@ class SolverT : public Solver {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         if (!st) return st;
+        if (best_kind == 0) return Status::err(DG_ERR_UNSUPPORTED, "no stage kernel supports this solve");
         lam_valid = false;  // set_dt cleared the accumulated wave speed
         {
             std::lock_guard<std::mutex> lk_(tune_mu());

@@ -29,28 +29,6 @@ __device__ __forceinline__ R face_phi(const TabP<R, P>& T, int f, int m, int j, 
     return rev ? T.S[f][m][NQ1 - 1 - j] : T.S[f][m][j];
 }
 
-// Ghost state UR of point j of a boundary side from the left trace UL
-// (eval_left_right, conserv_kernels.cu:647-671): reflecting keeps rho and E
-// and the velocity's component along the wall (supersonic.cu:64-108, wall
-// normal precomputed per point); inflow and outflow take the problem's
-// state at the point (euler.cu:177-241, precomputed in FP64).
-template <int N, int NQ1, typename R>
-__device__ __forceinline__ void boundary_ghost(const double* bnd, int w, int j, const R* UL, R* UR) {
-    const double* g = bnd + ((size_t)(w & NB_ELEM_MASK) * NQ1 + j) * 4;
-    if (((w >> 28) & 3) == BC_REFLECTING) {
-        UR[0] = UL[0];
-        if constexpr (N == 4) {
-            UR[3] = UL[3];
-            const R Nx = (R)g[0], Ny = (R)g[1];
-            UR[1] = (UL[1] * Ny - UL[2] * Nx) * Ny;
-            UR[2] = -(UL[1] * Ny - UL[2] * Nx) * Nx;
-        }
-    } else {
-#pragma unroll
-        for (int v = 0; v < N; v++) UR[v] = (R)g[v];
-    }
-}
-
 // Processes element e (clamped for loads when invalid, no stores then);
 // returns its cell wave speed after the last stage of a step, else 0.
 template <int N, int P, typename R>

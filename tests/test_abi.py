@@ -142,9 +142,16 @@ def test_read_reference_boundary_mesh():
     with pytest.raises(dg.DGError) as e:
         dg.Solver(m, dg.supersonic_vortex(), 1)
     assert e.value.code == dg.STATUS_CODES["DG_ERR_HIP"]
-    # boundary sides are supported up to p = 3, refused before any device work
+    # boundary sides: p <= 3 on the lane kernel, Euler p <= 4 on the
+    # matrix-core kernels; beyond that refused before any device work
     with pytest.raises(dg.DGError) as e:
         dg.Solver(m, dg.supersonic_vortex(), 4)
+    assert e.value.code == dg.STATUS_CODES["DG_ERR_HIP"]
+    with pytest.raises(dg.DGError) as e:
+        dg.Solver(m, dg.supersonic_vortex(), 5)
+    assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
+    with pytest.raises(dg.DGError) as e:
+        dg.Solver(m, dg.advection_gaussian(), 4)
     assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
 
 

@@ -5,11 +5,13 @@ outflow rays), through the C ABI against the FP64 oracle.
 Bars: FP64 within 1e-13 relative max-norm per RHS and 1e-12 relative L2
 over a run with the same step count; FP32 within 1e-4 per RHS (a
 near-steady state: k is a small difference of O(1) fluxes) and 1e-5 over a
-run.  Boundary meshes run on the lane-per-element kernel up to p = 3; the
-matrix-core kernels and batched solves refuse them.  The first run of the
+run.  Boundary meshes run on the lane-per-element kernel up to p = 3 and,
+for Euler, on the matrix-core kernels (kinds 2 and 3, their boundary
+variant) up to p = 4; batched solves refuse them.  The first run of the
 p = 4 FP64 lane kernel with the boundary path faulted the GPU (memory
-aperture violation) and its cause is not found: p >= 4 is refused on
-boundary meshes (checked on the CPU in test_abi) and no GPU test runs it.
+aperture violation) and its cause is not found: the lane kernel refuses
+boundary meshes at p >= 4 (checked here and on the CPU in test_abi) and no
+GPU test runs it.
 """
 import numpy as np
 import pytest
@@ -32,16 +34,35 @@ def test_boundary_rhs_parity(p, prec, quirks):
     u = perturbed_state(o, prob, m.period, seed=p, amp=1e-3)
     ref = o.rhs(u, dt=0.01)
     s = dg.Solver(m, prob, p, prec)
-    assert s.kernel() == 1
-    got = s.rhs(u, dt=0.01)
-    err = rel_max(got, ref)
-    print("  rel max err %.3e" % err)
-    # FP32: the state is near the steady solution, so k is a small difference
-    # of O(1) fluxes and its relative rounding error is some 300x FP32's
-    assert err <= (1e-13 if prec == dg.DG_FP64 else 1e-4)
+    for kind in (1, 2, 3):
+        s.set_kernel(kind)
+        err = rel_max(s.rhs(u, dt=0.01), ref)
+        print("  kernel %d rel max err %.3e" % (kind, err))
+        # FP32: the state is near the steady solution, so k is a small
+        # difference of O(1) fluxes and its relative rounding error is some
+        # 300x FP32's
+        assert err <= (1e-13 if prec == dg.DG_FP64 else 1e-4)
 
 
-@pytest.mark.parametrize("p", [1, 3])
+@pytest.mark.parametrize("quirks", [0, 15])
+@pytest.mark.parametrize("prec", [dg.DG_FP64, dg.DG_FP32])
+def test_boundary_rhs_parity_p4_matrix_cores(prec, quirks):
+    print("boundary rhs p=4 prec=%d quirks=%d" % (prec, quirks), flush=True)
+    m = golden_mesh("sv1refined")
+    prob = dg.supersonic_vortex(quirks=quirks)
+    o = orc.Oracle.from_mesh(m, 4, prob)
+    u = perturbed_state(o, prob, m.period, seed=4, amp=1e-3)
+    ref = o.rhs(u, dt=0.01)
+    s = dg.Solver(m, prob, 4, prec)
+    assert s.kernel() in (2, 3)
+    for kind in (2, 3):
+        s.set_kernel(kind)
+        err = rel_max(s.rhs(u, dt=0.01), ref)
+        print("  kernel %d rel max err %.3e" % (kind, err))
+        assert err <= (1e-13 if prec == dg.DG_FP64 else 1e-4)
+
+
+@pytest.mark.parametrize("p", [1, 3, 4])
 def test_boundary_rhs_parity_face1(p):
     """Boundary sides on local face 1 (vertices relabelled)."""
     m = relabelled(golden_mesh("sv1"))
@@ -49,12 +70,14 @@ def test_boundary_rhs_parity_face1(p):
     o = orc.Oracle.from_mesh(m, p, prob)
     u = perturbed_state(o, prob, m.period, seed=5, amp=1e-3)
     ref = o.rhs(u, dt=0.01)
-    got = dg.Solver(m, prob, p, dg.DG_FP64).rhs(u, dt=0.01)
-    assert rel_max(got, ref) <= 1e-13
+    s = dg.Solver(m, prob, p, dg.DG_FP64)
+    for kind in ((1, 2) if p <= 3 else (2,)):
+        s.set_kernel(kind)
+        assert rel_max(s.rhs(u, dt=0.01), ref) <= 1e-13, kind
 
 
 @pytest.mark.parametrize("prec", [dg.DG_FP64, dg.DG_FP32])
-@pytest.mark.parametrize("p", [2, 3])
+@pytest.mark.parametrize("p", [2, 3, 4])
 def test_boundary_run_vs_oracle(p, prec):
     m = golden_mesh("sv1refined")
     prob = dg.supersonic_vortex()
@@ -65,7 +88,7 @@ def test_boundary_run_vs_oracle(p, prec):
     s.project_ic()
     n, t = s.integrate(0.03, 0.3)
     err = orc.rel_l2(s.get_state(), ref)
-    print("p=%d prec=%d steps %d/%d rel L2 %.3e" % (p, prec, n, n_ref, err))
+    print("p=%d prec=%d kernel %d steps %d/%d rel L2 %.3e" % (p, prec, s.kernel(), n, n_ref, err))
     assert t == 0.03
     if prec == dg.DG_FP64:
         assert n == n_ref and err <= 1e-12
@@ -77,9 +100,12 @@ def test_boundary_mesh_kernel_choice():
     m = golden_mesh("sv1")
     prob = dg.supersonic_vortex()
     s = dg.Solver(m, prob, 3, dg.DG_FP32)
-    assert s.kernel() == 1  # the autotuner's only candidate
+    assert s.kernel() in (1, 2, 3)
+    s.set_kernel(2)
+    s4 = dg.Solver(m, prob, 4, dg.DG_FP32)
+    assert s4.kernel() in (2, 3)
     with pytest.raises(dg.DGError) as e:
-        s.set_kernel(2)
+        s4.set_kernel(1)  # the lane kernel takes boundary sides up to p = 3
     assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
     b = dg.Batch(dg.DG_FP32)
     with pytest.raises(dg.DGError) as e:
