@@ -148,7 +148,8 @@ enum { BC_REFLECTING = 1, BC_OUTFLOW = 2, BC_INFLOW = 3 };
 // Highest order whose lane kernel carries the boundary path.  The first
 // run of the p = 4 FP64 instance with it faulted the GPU (memory aperture
 // violation) and the cause is not found, so p >= 4 keeps the kernel without
-// it and the solver refuses boundary meshes there.
+// it; boundary meshes at p = 4 run on the matrix-core kernels' boundary
+// variant (Euler), and p = 5 is refused.
 constexpr int VALU_BOUNDARY_MAX_P = 3;
 
 enum StageKind {

@@ -187,7 +187,9 @@ int dg_tables(int p, int* sizes, double* r1, double* r2, double* w, double* s_r,
  * inflow and outflow the problem's initial state at the face point
  * (euler.cu:177-241), reflecting the interior density and energy and the
  * velocity's component along the wall whose normal is the radial direction
- * (supersonic.cu:64-108).  Such meshes run on the lane-per-element kernel. */
+ * (supersonic.cu:64-108).  Such meshes run on the lane-per-element kernel
+ * up to p = 3 and, for Euler, on the matrix-core kernels up to p = 4; other
+ * orders and physics on them return DG_ERR_UNSUPPORTED, as do batches. */
 int dg_solver_create(const dg_mesh* m, const dg_problem_desc* prob, int p, int precision, dg_solver** out);
 void dg_solver_destroy(dg_solver* s);
 /* sizes[0..5] = num_elem, num_sides, n_p, n_q, n_q1, N */
