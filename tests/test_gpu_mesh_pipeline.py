@@ -1,7 +1,8 @@
 """GPU parity on meshes read from gmsh files: a periodic box paired by the
 converter (every kernel, including the matrix-core ones at p >= 3) and a
-channel periodic in x with inflow / outflow sides (lane kernel, p <= 3),
-through the C ABI against the FP64 oracle.  Bars as for the generated
+channel periodic in x with inflow / outflow sides (lane kernel up to p = 3,
+the matrix-core kernels' boundary variant up to p = 4), through the C ABI
+against the FP64 oracle.  Bars as for the generated
 meshes: FP64 within 1e-13 relative max-norm per RHS, FP32 within 1e-5."""
 import pytest
 
@@ -31,7 +32,7 @@ def test_periodic_gmsh_rhs_parity(tmp_path, p, prec):
         assert err <= (1e-13 if prec == dg.DG_FP64 else 1e-5)
 
 
-@pytest.mark.parametrize("p", [1, 3])
+@pytest.mark.parametrize("p", [1, 3, 4])
 def test_channel_gmsh_rhs_parity(tmp_path, p):
     print("channel gmsh rhs p=%d" % p, flush=True)
     V = generated(12, 8).arrays()[0]
@@ -42,5 +43,10 @@ def test_channel_gmsh_rhs_parity(tmp_path, p):
     prob = dg.euler_vortex(vortices=((1.5, 1.0, 2.0),))
     o = orc.Oracle.from_mesh(m, p, prob)
     u = perturbed_state(o, prob, m.period, seed=p, amp=1e-2)
-    got = dg.Solver(m, prob, p, dg.DG_FP64).rhs(u, dt=0.01)
-    assert rel_max(got, o.rhs(u, dt=0.01)) <= 1e-13
+    ref = o.rhs(u, dt=0.01)
+    s = dg.Solver(m, prob, p, dg.DG_FP64)
+    for kind in ([1, 2] if p <= 3 else [2]):
+        s.set_kernel(kind)
+        err = rel_max(s.rhs(u, dt=0.01), ref)
+        print("  kernel %d rel max err %.3e" % (kind, err))
+        assert err <= 1e-13
