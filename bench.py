@@ -14,9 +14,11 @@ replays the captured step graph `--steps` times with the state resident in
 HBM.
 """
 import argparse
+import hashlib
 import json
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -64,26 +66,45 @@ def kernel_bytes_per_elem(N, n_p, s, kernel):
     return 4.25 * N * n_p * s + 14 * s + 3 * 4
 
 
-def cpu_baseline(p, seed, n_patch=64, steps=40):
-    """FP64 serial oracle on a 64x64-cell patch of the same workload (same
-    domain size per cell, same vortex superposition), 1 thread."""
-    import dgcuda_amd as dg
-    import oracle as orc
+def start_cpu_baseline(c3_steps):
+    """The FP64 serial oracle on one pinned host core (oracle/cpu_baseline.py)
+    in a child process, started before this process touches the GPU and run
+    beside the GPU measurements."""
+    return subprocess.Popen([sys.executable, "-m", "oracle.cpu_baseline", "--c3-steps", str(c3_steps)], cwd=ROOT,
+                            stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
 
-    h = 10.0 / 512
-    L = n_patch * h
-    vs = tuple((b, x * L / 10.0, y * L / 10.0) for b, x, y in vortices(seed))
-    m = dg.Mesh.periodic_jittered(n_patch, n_patch, 0.0, L, 0.0, L, 0.1, seed)
-    prob = dg.euler_vortex(vs)
-    o = orc.Oracle.from_mesh(m, p, prob)
-    u = o.project_ic(orc.ic_array(prob, m.period))
-    t0 = time.perf_counter()
-    u, _ = o.run_steps(u, steps, 0.3)
-    el = time.perf_counter() - t0
-    dof = m.num_elem * o.n_p * o.N
-    return dict(value=dof * 4 * steps / el, unit="DOF*stage/s", cores=1, kind="port",
-                sample="FP64 serial oracle, %dx%d-cell patch (%d triangles) of the workload, p=%d, %d RK4 steps, "
-                       "%.1f s" % (n_patch, n_patch, m.num_elem, p, steps, el))
+
+def finish_cpu_baseline(proc):
+    out, err = proc.communicate(timeout=1800)
+    if proc.returncode != 0:
+        raise RuntimeError("CPU baseline failed (exit %d): %s" % (proc.returncode, err[-2000:]))
+    d = json.loads(out.strip().splitlines()[-1])
+    c3, c1, pl = d["c3"], d["c1"], d["c3_plain"]
+    sample = ("FP64 serial oracle, 1 thread pinned to core %d of %d (%s): %d full RK4 steps of this workload "
+              "(%d triangles, p=4) in %.1f s, 200 steps extrapolated to %.0f s"
+              % (d["pinned_core"], d["host_logical_cores"], d["cpu_model"], c3["steps"], c3["triangles"],
+                 c3["seconds"], c3["extrapolated_200_steps_s"]))
+    return dict(value=c3["value"], unit="DOF*stage/s", cores=1, kind="port", sample=sample,
+                cpu_model=d["cpu_model"], host_logical_cores=d["host_logical_cores"], pinned_core=d["pinned_core"],
+                extrapolated_200_steps_s=c3["extrapolated_200_steps_s"],
+                advection_p1_2048_full_run=dict(value=c1["value"], steps=c1["steps"], seconds=c1["seconds"]),
+                reference_structured=dict(value=pl["value"], seconds=pl["seconds"],
+                                          sample="the same oracle without hoisting (traces and fluxes recomputed "
+                                                 "per mode, as the reference's loops do), %d RK4 steps on a "
+                                                 "64x64-cell patch (%d triangles) of the workload"
+                                                 % (pl["steps"], pl["triangles"])))
+
+
+# stage-kernel sources whose change makes a recorded traffic profile stale
+STAGE_SOURCES = ["dg_mfma_body.h", "dg_mfma.hip", "dg_device.h", "dg_kernels.h"]
+
+
+def stage_sources_sha256():
+    h = hashlib.sha256()
+    for name in STAGE_SOURCES:
+        with open(os.path.join(ROOT, "dgcuda_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
 
 
 def load_traffic(args, kernel):
@@ -97,7 +118,11 @@ def load_traffic(args, kernel):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get("bytes_per_launch") if d.get("kernel") == kernel else None
+    if d.get("kernel") != kernel:
+        return None
+    if d.get("sources_sha256") != stage_sources_sha256():
+        return "stale"  # profiled on other stage-kernel sources
+    return d.get("bytes_per_launch")
 
 
 def batch_summary(steps=20, steps64=4):
@@ -150,6 +175,11 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 VALU, 2 matrix cores")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-c3-steps", type=int, default=2, help="full RK4 steps of the CPU baseline sample")
+    ap.add_argument("--prewarm-s", type=float, default=2.0,
+                    help="seconds of graph-replayed steps before the warm-up (lets the clocks settle)")
+    ap.add_argument("--accuracy-steps", type=int, default=200,
+                    help="steps of the FP32 vs FP64 accuracy check (separate solves from the initial state)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the ragged-batch measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurements")
@@ -162,6 +192,10 @@ def main():
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    cpu_proc = None
+    if not args.no_cpu and not args.profile_steps:
+        cpu_proc = start_cpu_baseline(args.cpu_c3_steps)
 
     import dgcuda_amd as dg
 
@@ -185,6 +219,14 @@ def main():
 
     K, n_p, N = s.num_elem, s.n_p, s.N
     dof = K * n_p * N
+    # pre-warm: replayed steps until the clocks have settled, then the
+    # requested warm-up; neither is timed
+    t_pw, n_pw = time.perf_counter(), 0
+    while time.perf_counter() - t_pw < args.prewarm_s:
+        s.run_steps(20, 0.3, use_graph=True)
+        s.sync()
+        n_pw += 20
+    prewarm = dict(seconds=round(time.perf_counter() - t_pw, 3), steps=n_pw)
     s.run_steps(args.warmup, 0.3, use_graph=True)
     s.sync()
     # timed region: the captured step graph replayed `steps` times, with HIP
@@ -196,7 +238,6 @@ def main():
     el = time.perf_counter() - t0
     value = dof * 4 * args.steps / el
     t_reached = s.audit()["t"]
-    state32 = None if args.no_accuracy else s.get_state()
 
     # dominant kernel: the fused stage kernel.  Its live duration comes from
     # the timed region's events: ms per replayed step, less the dt kernel
@@ -219,26 +260,48 @@ def main():
                 kernel_bytes_GBs=round(kbytes_launch / (stage_ms * 1e-3) / 1e9, 1),
                 kernel_bytes_frac=round(kbytes_launch / (stage_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4))
 
+    kernel_kind = s.kernel()
+    s.close()  # free the workload before the accuracy, batch and serving runs
     extra = {}
-    if state32 is not None and prec == dg.DG_FP32:
-        s64 = dg.Solver(mesh, prob, args.p, dg.DG_FP64)
-        s64.project_ic()
-        n64, _ = s64.integrate(t_reached, 0.3)
+    if not args.no_accuracy and prec == dg.DG_FP32:
+        # accuracy of the benchmarked configuration: FP32 for accuracy-steps
+        # CFL steps from the projected state, against FP64 to the same time,
+        # and an FP64 run with FP32-rounded geometry (the geometry storage's
+        # share of the FP32 error)
         import oracle as orc
 
-        err = orc.rel_l2(state32, s64.get_state())
-        extra["rel_l2_fp32_vs_fp64"] = float("%.3e" % err)
-        extra["accuracy_reference"] = ("GPU FP64 run to the same time (%d steps); the FP64 GPU path matches the "
-                                       "FP64 oracle to <=1e-12 in tests" % n64)
-        s64.close()
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(args.p, args.seed)
-    kernel_kind = s.kernel()
-    s.close()  # free the workload before the batch and serving runs
+        a32 = dg.Solver(mesh, prob, args.p, dg.DG_FP32)
+        a32.set_kernel(kernel_kind)
+        a32.project_ic()
+        a32.run_steps(args.accuracy_steps, 0.3)
+        t_acc = a32.audit()["t"]
+        st32 = a32.get_state()
+        a32.close()
+        a64 = dg.Solver(mesh, prob, args.p, dg.DG_FP64)
+        a64.project_ic()
+        n64, _ = a64.integrate(t_acc, 0.3)
+        st64 = a64.get_state()
+        a64.close()
+        g64 = dg.Solver(mesh, prob, args.p, dg.DG_FP64 | dg.DG_GEOM_FP32)
+        g64.project_ic()
+        g64.integrate(t_acc, 0.3)
+        stg = g64.get_state()
+        g64.close()
+        extra["accuracy"] = dict(
+            steps=args.accuracy_steps, t=t_acc, fp64_steps=n64,
+            rel_l2_fp32_vs_fp64=float("%.3e" % orc.rel_l2(st32, st64)),
+            rel_l2_fp32_geometry_only=float("%.3e" % orc.rel_l2(stg, st64)),
+            reference="GPU FP64 run to the same time; the FP64 GPU path matches the FP64 oracle to <=1e-12 in "
+                      "tests. rel_l2_fp32_geometry_only: FP64 state and kernels with the geometry rounded "
+                      "through FP32, as the FP32 path stores it")
     batch = None if args.no_batch else batch_summary()
     serving = None if args.no_serving else serving_summary()
-    bw_ms, bw = dg.bandwidth_probe(1 << 30, 10)
+    bw_ms, bw, bw_kernel = dg.bandwidth_probe(1 << 30, 10, with_variant=True)
+    cpu = finish_cpu_baseline(cpu_proc) if cpu_proc is not None else None
+    traffic = roof["traffic"]
+    stale = traffic == "stale"
+    if stale:
+        roof["traffic"] = traffic = None
     out = {
         "metric": "Euler p=4 DOF*stage updates/s",
         "value": value,
@@ -254,13 +317,17 @@ def main():
         "data": "synthetic: seeded 3-vortex isentropic superposition on a seeded jittered periodic mesh",
         "config": {"workload": "euler_rk4_p%d_3vortex_%dx%d_periodic" % (args.p, args.n, args.n),
                    "triangles": K, "p": args.p, "n_p": n_p, "dof": dof, "cfl": 0.3, "integrator": "rk4",
-                   "kernel": kernel_kind, "geometry": "fp64-computed"},
+                   "kernel": kernel_kind,
+                   "geometry": "fp64-computed, fp32-stored" if prec == dg.DG_FP32 else "fp64"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(bw, 1),
+        "copy_kernel": bw_kernel,
         # measured HBM traffic per launch over the live duration, against the copy probe
-        "traffic_GBs": None if roof["traffic"] is None else round(roof["traffic"] / (stage_ms * 1e-3) / 1e9, 1),
-        "traffic_frac_of_copy": None if roof["traffic"] is None else round(roof["traffic"] / (stage_ms * 1e-3) / 1e9 / bw, 4),
+        "traffic_GBs": None if traffic is None else round(traffic / (stage_ms * 1e-3) / 1e9, 1),
+        "traffic_frac_of_copy": None if traffic is None else round(traffic / (stage_ms * 1e-3) / 1e9 / bw, 4),
+        "traffic_stale": stale,
+        "prewarm": prewarm,
         "setup_s": round(setup_s, 3),
         "t_reached": t_reached,
         "host": platform.node(),

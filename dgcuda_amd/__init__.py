@@ -25,7 +25,7 @@ LIB_PATH = os.environ.get("DGMI_LIB") or os.path.join(_HERE, "libdgmi.so")
 
 DG_OK = 0
 DG_PHYS_ADVECTION, DG_PHYS_EULER = 1, 4
-DG_FP64, DG_FP32 = 0, 1
+DG_FP64, DG_FP32, DG_GEOM_FP32 = 0, 1, 0x100
 DG_RK4, DG_RK2 = 1, 2
 DG_IC_CONSTANT, DG_IC_GAUSSIAN, DG_IC_VORTEX, DG_IC_SUPERSONIC = 0, 1, 2, 3
 DG_QUIRK_FLOAT_SQRT, DG_QUIRK_FLOAT_GEOM, DG_QUIRK_FLOAT_POW, DG_QUIRK_REFLECT_MIRROR = 1, 2, 4, 8
@@ -108,9 +108,12 @@ def lib():
             "dg_integrate": (I, [VP, D, D, I, P(I64), dp]), "dg_run_steps": (I, [VP, I, D, I, I]),
             "dg_audit": (I, [VP, P(AuditOut)]), "dg_eval_vertices": (I, [VP, I, dp]),
             "dg_solver_sync": (I, [VP]), "dg_solver_set_kernel": (I, [VP, I]), "dg_solver_get_kernel": (I, [VP, ip]),
+            "dg_solver_set_max_groups": (I, [VP, I]), "dg_solver_launch_info": (I, [VP, ip]),
+            "dg_batch_set_max_groups": (I, [VP, I]), "dg_batch_launch_info": (I, [VP, ip]),
             "dg_time_stage_kernels": (I, [VP, I, D, dp]), "dg_time_step_graph": (I, [VP, I, D, dp]),
             "dg_solver_device_bytes": (I, [VP, P(I64)]),
-            "dg_bandwidth_probe": (I, [I64, I, dp]), "dg_selftest_mfma": (I, [dp]),
+            "dg_bandwidth_probe": (I, [I64, I, dp]), "dg_bandwidth_probe_ex": (I, [I64, I, dp, I]),
+            "dg_selftest_mfma": (I, [dp]),
             "dg_write_msh": (I, [VP, I, dp, C.c_char_p]),
             "dg_l2_error": (I, [VP, P(ProblemDesc), I, dp, dp]),
             "dg_batch_create": (I, [I, P(VP)]), "dg_batch_destroy": (None, [VP]),
@@ -153,11 +156,12 @@ def device_count():
     return lib().dg_device_count()
 
 
-def bandwidth_probe(nbytes=1 << 30, iters=20):
-    """Streaming device copy: (ms per copy, GB/s counting read + write)."""
-    out = np.zeros(2)
-    _check(lib().dg_bandwidth_probe(nbytes, iters, _dp(out)))
-    return out[0], out[1]
+def bandwidth_probe(nbytes=1 << 30, iters=20, with_variant=False):
+    """Streaming device copy, the fastest of 4 copy kernels: (ms per copy, GB/s
+    counting read + write[, the kernel's number])."""
+    out = np.zeros(3)
+    _check(lib().dg_bandwidth_probe_ex(nbytes, iters, _dp(out), 3))
+    return (out[0], out[1], int(out[2])) if with_variant else (out[0], out[1])
 
 
 def n_modes(p):
@@ -306,7 +310,7 @@ class Solver:
         _check(lib().dg_solver_info(self._h, _ip(sizes), C.byref(r)))
         self.num_elem, self.num_sides, self.n_p, self.n_q, self.n_q1, self.N = (int(x) for x in sizes)
         self.min_r = r.value
-        self.p, self.precision = p, precision
+        self.p, self.precision = p, precision & ~DG_GEOM_FP32
 
     @property
     def shape(self):
@@ -370,6 +374,16 @@ class Solver:
         _check(lib().dg_solver_get_kernel(self._h, C.byref(k)))
         return k.value
 
+    def set_max_groups(self, n):
+        """Cap the persistent matrix-core grid at n workgroups (0 = all resident)."""
+        _check(lib().dg_solver_set_max_groups(self._h, n))
+
+    def launch_info(self):
+        """dict(kind, tiles, groups, walk): walk = most tiles one workgroup processes."""
+        o = np.zeros(4, np.int32)
+        _check(lib().dg_solver_launch_info(self._h, _ip(o)))
+        return dict(kind=int(o[0]), tiles=int(o[1]), groups=int(o[2]), walk=int(o[3]))
+
     def time_stage_kernels(self, nsteps, cfl):
         out = np.zeros(4)
         _check(lib().dg_time_stage_kernels(self._h, nsteps, cfl, _dp(out)))
@@ -430,6 +444,16 @@ class Batch:
         _check(lib().dg_batch_audit(self._h, slot, C.byref(a)))
         return dict(mass=list(a.mass), min_density=a.min_density, min_pressure=a.min_pressure, n_bad=a.n_bad,
                     t=a.t, steps=a.steps)
+
+    def set_max_groups(self, n):
+        """Cap the persistent grid at n workgroups (0 = all resident)."""
+        _check(lib().dg_batch_set_max_groups(self._h, n))
+
+    def launch_info(self):
+        """dict(groups, tile_items, lane_items, longest_run) of the uploaded work list."""
+        o = np.zeros(4, np.int32)
+        _check(lib().dg_batch_launch_info(self._h, _ip(o)))
+        return dict(groups=int(o[0]), tile_items=int(o[1]), lane_items=int(o[2]), longest_run=int(o[3]))
 
     def info(self):
         n, e, b = C.c_int(), C.c_int64(), C.c_int64()

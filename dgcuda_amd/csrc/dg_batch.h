@@ -24,6 +24,12 @@ class BatchEngine {
     virtual Status sync() = 0;
     virtual hipStream_t stream() = 0;
     virtual int max_tile_halo() const = 0;
+    // caps the persistent grid at max_groups workgroups (a multiple of 8, at
+    // least 8; 0 = every resident workgroup)
+    virtual Status set_max_groups(int max_groups) = 0;
+    // out[0] workgroups launched, out[1] tile items, out[2] lane items,
+    // out[3] longest run of consecutive tiles of one solve a workgroup takes
+    virtual void launch_info(int* out) const = 0;
 };
 
 // a solver set up for batched stepping (FP32: matrix-core tiling at every p)
@@ -40,6 +46,8 @@ class Batch {
     virtual Status get_state(int slot, double* c_ref) = 0;
     virtual Status audit(int slot, dg_audit_out* out) = 0;
     virtual void info(int* n_solves, int64_t* total_elems, int64_t* device_bytes) const = 0;
+    virtual Status set_max_groups(int max_groups) = 0;
+    virtual void launch_info(int* out) const = 0;
 };
 
 }  // namespace dgmi

@@ -313,6 +313,9 @@ class EngineT : public BatchEngine {
     double* d_status = nullptr;
     int cap_slots = 0, cap_items = 0;
     int nslots = 0, nitems = 0, grid = 0, dt_grid = 0;
+    int grid_resident = 0;           // workgroups that fit on the device at once
+    std::vector<int2> h_items;       // host copy of the work list and its XCD bounds
+    std::vector<int> h_parts;
     size_t lds = 0;
     hipGraphExec_t gexec = nullptr;
     int solo_min = 0;             // 0: every solve in the batch launch
@@ -347,7 +350,7 @@ class EngineT : public BatchEngine {
         HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BatchKernel<R, ST_RK4_2>::fn(), 256, lds));
         HIPST(hipGetDevice(&dev));
         HIPST(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid = std::max(8, per_cu * cus / 8 * 8);
+        grid = grid_resident = std::max(8, per_cu * cus / 8 * 8);
         Status s1 = reserve(max_slots, max_items);
         if (!s1) return s1;
         HIPST(hipStreamSynchronize(st));
@@ -461,6 +464,8 @@ class EngineT : public BatchEngine {
         HIPST(hipStreamSynchronize(st));  // the host vectors go out of scope
         nslots = ns;
         nitems = ni;
+        h_items = hi;
+        h_parts = xs;
         // The solo launches are part of the captured step, with the buffers of
         // the solves they were captured for; compare by id, since a new solve
         // can sit at the address of a deleted one.
@@ -534,6 +539,38 @@ class EngineT : public BatchEngine {
     }
     hipStream_t stream() override { return st; }
     int max_tile_halo() const override { return MFMA_MAX_HALO; }
+
+    Status set_max_groups(int n) override {
+        if (n < 0 || (n > 0 && n < 8)) return Status::err(DG_ERR_ARG, "max_groups must be 0 or at least 8");
+        const int g = n == 0 ? grid_resident : std::min(grid_resident, n / 8 * 8);
+        if (g != grid) drop_graph();
+        grid = g;
+        return Status::ok();
+    }
+
+    // Workgroup bi of XCD x takes tile items parts[9 + x] + bi, + G8, ...;
+    // a run is a stretch of those that belong to one solve (batch_mfma_run).
+    void launch_info(int* o) const override {
+        o[0] = grid;
+        o[1] = o[2] = o[3] = 0;
+        if (h_parts.size() < (size_t)BATCH_PARTS) return;
+        o[2] = h_parts[8] - h_parts[0];
+        o[1] = h_parts[BATCH_TILE_PART + 8] - h_parts[BATCH_TILE_PART];
+        const int G8 = grid / 8;
+        int longest = 0;
+        for (int x = 0; x < 8; x++) {
+            const int i0 = h_parts[BATCH_TILE_PART + x], i1 = h_parts[BATCH_TILE_PART + x + 1];
+            for (int bi = 0; bi < G8; bi++) {
+                int run = 0, slot = -1;
+                for (int it = i0 + bi; it < i1; it += G8) {
+                    run = (h_items[it].x == slot) ? run + 1 : 1;
+                    slot = h_items[it].x;
+                    longest = std::max(longest, run);
+                }
+            }
+        }
+        o[3] = longest;
+    }
 };
 
 Status BatchEngine::create(int precision, int max_slots, int max_items, BatchEngine** out) {
@@ -670,6 +707,10 @@ class BatchImpl : public Batch {
         if (!st) return st;
         return solvers[slot]->audit(out);
     }
+
+    Status set_max_groups(int n) override { return eng->set_max_groups(n); }
+    // the work list is uploaded at the next run; describe that one
+    void launch_info(int* out) const override { eng->launch_info(out); }
 
     void info(int* n, int64_t* elems, int64_t* dbytes) const override {
         int64_t e = 0, b = 0;

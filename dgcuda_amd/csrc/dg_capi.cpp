@@ -391,6 +391,18 @@ int dg_solver_get_kernel(const dg_solver* s, int* kind) {
     return DG_OK;
 }
 
+int dg_solver_set_max_groups(dg_solver* s, int max_groups) {
+    SOLVER_CHECK(s);
+    GUARD_BEGIN return ret(s->s->set_max_groups(max_groups)); GUARD_END
+}
+
+int dg_solver_launch_info(const dg_solver* s, int* out4) {
+    SOLVER_CHECK(s);
+    if (!out4) return fail(DG_ERR_ARG, "null output");
+    s->s->launch_info(out4);
+    return DG_OK;
+}
+
 int dg_time_stage_kernels(dg_solver* s, int nsteps, double cfl, double* out) {
     SOLVER_CHECK(s);
     if (nsteps <= 0 || !out) return fail(DG_ERR_ARG, "invalid argument");
@@ -409,7 +421,9 @@ int dg_solver_device_bytes(const dg_solver* s, int64_t* bytes) {
     return DG_OK;
 }
 
-int dg_bandwidth_probe(int64_t bytes, int iters, double* out) {
+int dg_bandwidth_probe(int64_t bytes, int iters, double* out) { return dg_bandwidth_probe_ex(bytes, iters, out, 2); }
+
+int dg_bandwidth_probe_ex(int64_t bytes, int iters, double* out, int nout) {
     GUARD_BEGIN
     if (bytes <= 0 || bytes % 16 || iters <= 0 || !out) return fail(DG_ERR_ARG, "invalid argument");
     Status st = ensure_device_ready();
@@ -430,17 +444,26 @@ int dg_bandwidth_probe(int64_t bytes, int iters, double* out) {
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, s);
-    if (e == hipSuccess) e = launch_stream_copy(a, b, bytes, s);  // warm-up
-    if (e == hipSuccess) e = hipEventRecord(e0, s);
-    for (int i = 0; i < iters && e == hipSuccess; i++) e = launch_stream_copy(a, b, bytes, s);
-    if (e == hipSuccess) e = hipEventRecord(e1, s);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    float best = 0;
+    int best_v = 0;
+    for (int v = 0; v < STREAM_COPY_VARIANTS && e == hipSuccess; v++) {
+        e = launch_stream_copy(a, b, bytes, s, v);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int i = 0; i < iters && e == hipSuccess; i++) e = launch_stream_copy(a, b, bytes, s, v);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && (v == 0 || ms < best)) {
+            best = ms;
+            best_v = v;
+        }
+    }
     cleanup();
     if (e != hipSuccess) return ret(hip_status(e, "bandwidth probe"));
-    out[0] = ms / iters;
+    out[0] = best / iters;
     out[1] = 2.0 * (double)bytes / (out[0] * 1e-3) / 1e9;
+    if (nout >= 3) out[2] = best_v;
     return DG_OK;
     GUARD_END
 }
@@ -537,6 +560,18 @@ int dg_batch_audit(dg_batch* b, int slot, dg_audit_out* out) {
 int dg_batch_info(const dg_batch* b, int* n_solves, int64_t* total_elems, int64_t* device_bytes) {
     BATCH_CHECK(b);
     b->b->info(n_solves, total_elems, device_bytes);
+    return DG_OK;
+}
+
+int dg_batch_set_max_groups(dg_batch* b, int max_groups) {
+    BATCH_CHECK(b);
+    GUARD_BEGIN return ret(b->b->set_max_groups(max_groups)); GUARD_END
+}
+
+int dg_batch_launch_info(const dg_batch* b, int* out4) {
+    BATCH_CHECK(b);
+    if (!out4) return fail(DG_ERR_ARG, "null output");
+    b->b->launch_info(out4);
     return DG_OK;
 }
 

@@ -79,7 +79,9 @@ hipError_t upload_tables();
 hipError_t upload_tables_kernels(const TabAll<double>& h64, const TabAll<float>& h32);
 hipError_t upload_tables_mfma(const TabAll<double>& h64, const TabAll<float>& h32);
 hipError_t upload_tables_batch(const TabAll<double>& h64, const TabAll<float>& h32);
-hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
+// streaming copy variants of the bandwidth probe (0..STREAM_COPY_VARIANTS-1)
+constexpr int STREAM_COPY_VARIANTS = 4;
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t st, int variant = 0);
 
 template <typename R>
 hipError_t launch_stage_valu(int N, int p, const StageArgs<R>& a, dim3 grid, hipStream_t st);

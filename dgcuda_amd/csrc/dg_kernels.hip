@@ -307,9 +307,30 @@ __global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
-hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
+// The same copy with four independent 16-byte loads in flight per thread
+// before their stores (64 bytes per thread and iteration).
+__global__ void __launch_bounds__(256) k_stream_copy4(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                     int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// variant 0: k_stream_copy on 1024 workgroups; 1..3: k_stream_copy4 on
+// 1024, 2048, 4096 workgroups
+hipError_t launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t st, int variant) {
     const int64_t n = bytes / 16;
-    hipLaunchKernelGGL(k_stream_copy, dim3(1024), dim3(256), 0, st, (const float4*)src, (float4*)dst, n);
+    if (variant == 0)
+        hipLaunchKernelGGL(k_stream_copy, dim3(1024), dim3(256), 0, st, (const float4*)src, (float4*)dst, n);
+    else
+        hipLaunchKernelGGL(k_stream_copy4, dim3(512 << variant), dim3(256), 0, st, (const float4*)src, (float4*)dst, n);
     return hipGetLastError();
 }
 

@@ -115,6 +115,12 @@ class Solver {
     virtual Status time_step_graph(int nsteps, double cfl, double* out) = 0;
     virtual Status set_kernel(int kind) = 0;
     virtual int kernel_kind() const = 0;
+    // caps the persistent matrix-core grid at max_groups workgroups (a
+    // multiple of 8, at least 8; 0 = every resident workgroup)
+    virtual Status set_max_groups(int max_groups) = 0;
+    // out[0] kernel kind, out[1] tiles, out[2] workgroups launched,
+    // out[3] most tiles one workgroup walks in a launch
+    virtual void launch_info(int* out) const = 0;
     virtual int64_t device_bytes() const = 0;
     virtual Status sync() = 0;
 };

@@ -275,6 +275,7 @@ class SolverT : public Solver {
     Ctrl hctrl{};
     LaunchShape shape;
     int kernel_req = 0;
+    int max_groups = 0;  // cap of the persistent matrix-core grid (0: every resident workgroup)
     bool lam_valid = false;
     hipGraphExec_t gexec = nullptr;
     int graph_scheme = 0;
@@ -286,6 +287,7 @@ class SolverT : public Solver {
     StateIx six{};
     bool state_set = false;  // c holds a state (projected, set or stepped)
     double b0 = 0;  // phi_0 at the first volume point (cell mean = c_0 * b0)
+    bool geom_fp32 = false;  // FP64 solve with the geometry rounded through FP32 (DG_GEOM_FP32)
 
     ~SolverT() override {
         if (stream) (void)hipStreamSynchronize(stream);  // nothing queued may use the buffers
@@ -367,18 +369,19 @@ class SolverT : public Solver {
         std::vector<R> he(5 * (size_t)Kp), hf(9 * (size_t)Kp);
         std::vector<int> hn(3 * (size_t)Kp);
         std::vector<double> hv(6 * (size_t)Kp), hj(Kp);
+        const auto gr = [this](double x) { return (R)(geom_fp32 ? (double)(float)x : x); };
         for (int i = 0; i < Kp; i++) {
             const int e = std::min(i, K - 1);
-            he[i] = (R)g.xr[e];
-            he[Kp + i] = (R)g.yr[e];
-            he[2 * (size_t)Kp + i] = (R)g.xs[e];
-            he[3 * (size_t)Kp + i] = (R)g.ys[e];
-            he[4 * (size_t)Kp + i] = (R)g.J[e];
+            he[i] = gr(g.xr[e]);
+            he[Kp + i] = gr(g.yr[e]);
+            he[2 * (size_t)Kp + i] = gr(g.xs[e]);
+            he[3 * (size_t)Kp + i] = gr(g.ys[e]);
+            he[4 * (size_t)Kp + i] = gr(g.J[e]);
             hj[i] = g.J[e];
             for (int f = 0; f < 3; f++) {
-                hf[(3 * f + 0) * (size_t)Kp + i] = (R)g.fnx[f][e];
-                hf[(3 * f + 1) * (size_t)Kp + i] = (R)g.fny[f][e];
-                hf[(3 * f + 2) * (size_t)Kp + i] = (R)g.flen[f][e];
+                hf[(3 * f + 0) * (size_t)Kp + i] = gr(g.fnx[f][e]);
+                hf[(3 * f + 1) * (size_t)Kp + i] = gr(g.fny[f][e]);
+                hf[(3 * f + 2) * (size_t)Kp + i] = gr(g.flen[f][e]);
                 hn[f * (size_t)Kp + i] = g.nbr[f][e];
             }
             for (int v = 0; v < 6; v++) hv[v * (size_t)Kp + i] = g.V[v][e];
@@ -498,6 +501,7 @@ class SolverT : public Solver {
                 // persistent: as many workgroups as fit on the device at once
                 const int res = mfma_resident_groups(N, p, mx.hcap, (int)sizeof(R), has_boundary);
                 if (res >= 8) s.grid = std::min(s.grid, res / 8 * 8);
+                if (max_groups >= 8) s.grid = std::min(s.grid, max_groups / 8 * 8);
             }
         }
         return Status::ok();
@@ -990,6 +994,36 @@ class SolverT : public Solver {
         return push_ctrl();
     }
     int kernel_kind() const override { return shape.kind; }
+
+    Status set_max_groups(int n) override {
+        if (n < 0 || (n > 0 && n < 8)) return Status::err(DG_ERR_ARG, "max_groups must be 0 or at least 8");
+        max_groups = n;
+        if (shape.kind != 2) return Status::ok();  // taken up when kind 2 is chosen
+        const int req = kernel_req;
+        Status st = choose_shape(2);
+        kernel_req = req;
+        if (!st) return st;
+        return push_ctrl();
+    }
+
+    // Kind 2 walks tiles tile0, tile0 + stride, ... inside its XCD's
+    // contiguous range of ceil(ntiles / 8) tiles (stride = grid / 8); the
+    // other kinds launch one workgroup per tile.
+    void launch_info(int* o) const override {
+        o[0] = shape.kind;
+        o[1] = shape.ntiles;
+        o[2] = shape.grid;
+        int walk = 1;
+        if (shape.grid > 0) {
+            if (shape.xcd_remap) {
+                const int t8 = (shape.ntiles + 7) / 8, stride = std::max(1, shape.grid / 8);
+                walk = (t8 + stride - 1) / stride;
+            } else {
+                walk = (shape.ntiles + shape.grid - 1) / shape.grid;
+            }
+        }
+        o[3] = walk;
+    }
     int64_t device_bytes() const override { return bytes + tile_bytes; }
     Status sync() override {
         HIPST(hipStreamSynchronize(stream));
@@ -998,10 +1032,14 @@ class SolverT : public Solver {
 };
 
 Status Solver::create(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out, int kernel) {
+    const bool geom32 = (precision & DG_GEOM_FP32) != 0;
+    precision &= ~DG_GEOM_FP32;
     Status st = check_problem(prob, p, precision);
     if (!st) return st;
+    if (geom32 && precision != DG_FP64) return Status::err(DG_ERR_ARG, "DG_GEOM_FP32 applies to DG_FP64 solves");
     if (precision == DG_FP64) {
         auto* s = new SolverT<double>();
+        s->geom_fp32 = geom32;
         st = s->init(m, prob, p, kernel);
         if (!st) {
             delete s;

@@ -57,8 +57,13 @@ typedef enum {
 
 /* physics: the value is the number of conserved variables (the reference's local_N) */
 enum { DG_PHYS_ADVECTION = 1, DG_PHYS_EULER = 4 };
-/* bulk arithmetic of the state and the kernels; geometry is always computed in FP64 */
-enum { DG_FP64 = 0, DG_FP32 = 1 };
+/* bulk arithmetic of the state and the kernels; geometry is always computed
+ * in FP64 and stored in the bulk precision.  DG_GEOM_FP32 may be or'ed into
+ * DG_FP64 for dg_solver_create: the geometric factors are then rounded
+ * through FP32 before use, which isolates the share of an FP32 run's error
+ * that FP32-stored geometry causes (a measurement aid; no counterpart in the
+ * reference). */
+enum { DG_FP64 = 0, DG_FP32 = 1, DG_GEOM_FP32 = 0x100 };
 /* time integrators, numbered as the reference's RK4 / RK2 (conserv.cu:19-20) */
 enum { DG_RK4 = 1, DG_RK2 = 2 };
 /* initial conditions; DG_IC_SUPERSONIC is the supersonic vortex between two
@@ -223,6 +228,17 @@ int dg_solver_sync(dg_solver* s);
  * per workgroup (2 and 3: FP32 or FP64, p = 1..4) */
 int dg_solver_set_kernel(dg_solver* s, int kind);
 int dg_solver_get_kernel(const dg_solver* s, int* kind);
+/* Caps the grid of the persistent matrix-core kernel (kind 2) at max_groups
+ * workgroups (rounded down to a multiple of 8; 0 = as many as are resident
+ * on the device, the default), so each workgroup walks more tiles.  Results
+ * do not depend on it.  Test and tuning aid; no counterpart in the reference,
+ * whose kernels launch one thread per element or side. */
+int dg_solver_set_max_groups(dg_solver* s, int max_groups);
+/* Launch shape of the stage kernels: out[0] kernel kind, out[1] tiles (64
+ * elements for kinds 2 and 3, 128 for kind 1), out[2] workgroups launched,
+ * out[3] the most tiles one workgroup processes in a launch.  No counterpart
+ * in the reference. */
+int dg_solver_launch_info(const dg_solver* s, int* out4);
 /* Times nsteps RK4 steps with HIP events on the solver's stream, one event
  * pair around every stage-kernel launch.  out[0] = mean stage-kernel ms,
  * out[1] = mean dt-kernel ms, out[2] = total ms, out[3] = number of stage
@@ -237,11 +253,17 @@ int dg_time_stage_kernels(dg_solver* s, int nsteps, double cfl, double* out);
 int dg_time_step_graph(dg_solver* s, int nsteps, double cfl, double* out);
 /* device buffer bytes held by the solver */
 int dg_solver_device_bytes(const dg_solver* s, int64_t* bytes);
-/* Device-to-device streaming copy of `bytes` (multiple of 16), iters times,
- * timed with HIP events: out[0] = mean ms per copy, out[1] = GB/s counting
- * read + write bytes.  The bandwidth reference of the roofline (no
- * counterpart in the reference). */
+/* Device-to-device streaming copy of `bytes` (multiple of 16), iters times
+ * for each of 4 copy kernels (one or four 16-byte loads in flight per thread,
+ * 1024 to 4096 workgroups), timed with HIP events; the fastest counts:
+ * out[0] = mean ms per copy, out[1] = GB/s counting read + write bytes.
+ * The bandwidth reference of the roofline (no counterpart in the
+ * reference). */
 int dg_bandwidth_probe(int64_t bytes, int iters, double* out);
+/* dg_bandwidth_probe with out[2] = the fastest copy kernel's number (0: one
+ * load per thread on 1024 workgroups, 1..3: four loads on 1024, 2048, 4096
+ * workgroups) when nout >= 3. */
+int dg_bandwidth_probe_ex(int64_t bytes, int iters, double* out, int nout);
 /* Runs one 16x16x4 FP32 and one FP64 matrix-core product on the GPU and
  * compares them with the host product under the lane layouts the kernels
  * assume; *max_err = max absolute difference (no counterpart in the
@@ -263,6 +285,16 @@ int dg_batch_integrate(dg_batch* b, double cfl, int use_graph, int64_t* steps);
 int dg_batch_get_state(dg_batch* b, int slot, double* c_ref);
 int dg_batch_audit(dg_batch* b, int slot, dg_audit_out* out);
 int dg_batch_info(const dg_batch* b, int* n_solves, int64_t* total_elems, int64_t* device_bytes);
+/* Caps the batch's persistent grid at max_groups workgroups (rounded down to
+ * a multiple of 8; 0 = every resident workgroup, the default).  Results do
+ * not depend on it.  Test and tuning aid; no counterpart in the reference. */
+int dg_batch_set_max_groups(dg_batch* b, int max_groups);
+/* Launch shape of the last uploaded work list (uploaded by the first run
+ * after dg_batch_add): out[0] workgroups, out[1] matrix-core tile items,
+ * out[2] lane items, out[3] the longest run of consecutive tiles of one solve
+ * that one workgroup takes through the pipelined tile body.  No counterpart
+ * in the reference. */
+int dg_batch_launch_info(const dg_batch* b, int* out4);
 
 /* ---------------- serving: continuous batching ---------------- */
 typedef struct {

@@ -115,3 +115,56 @@ def test_server_bad_request_fails_alone():
     res = {r.id: r for r in sv.poll(64)}
     assert res[bad_id].status == dg.STATUS_CODES["DG_ERR_NUMERIC"]
     assert all(res[i].status == 0 for i in good_ids)
+
+
+def c5_shaped_requests(n=10, seed=21, rate=40.0, kmin=1000, kmax=4000):
+    """The serving configuration's recipe (Poisson arrivals, log-uniform mesh
+    sizes, p in 1..3, one random vortex, end time uniform) at sizes and end
+    times the oracle finishes in seconds."""
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        K = float(np.exp(rng.uniform(np.log(kmin), np.log(kmax))))
+        nx = max(2, int(round(np.sqrt(K / 2.0))))
+        p = int(rng.integers(1, 4))
+        te = float(rng.uniform(0.05, 0.15))
+        prob = dg.euler_vortex(((float(rng.uniform(2, 5)), float(rng.uniform(0, 10)), float(rng.uniform(0, 10))),))
+        out.append(dg.request(nx, p, te, prob, mesh_seed=int(rng.integers(1, 2**31)), arrival_s=t))
+    return out
+
+
+def test_served_stream_vs_oracle():
+    """Every served FP32 final state of a C5-shaped stream, with the larger
+    requests on their own kernels inside the step graph and the others in the
+    batch launch, is within 1e-5 relative L2 of the FP64 oracle run of the
+    same request, and conserves mass to FP32 round-off."""
+    import oracle as orc
+
+    reqs = c5_shaped_requests()
+    sizes = [2 * r.nx * r.ny for r in reqs]
+    assert min(sizes) < 2000 <= max(sizes) and {r.p for r in reqs} == {1, 2, 3}
+    sv = dg.Server(dg.DG_FP32, max_slots=6, steps_per_round=8, cfl=0.3, solo_min_elems=2000)
+    sv.keep_states(True)
+    t0 = sv.now()
+    ids = []
+    for r in reqs:
+        r.arrival_s += t0
+        ids.append(sv.submit(r))
+    sv.drain()
+    res = {r.id: r for r in sv.poll(64)}
+    assert len(res) == len(reqs)
+    for k, r in enumerate(reqs):
+        out = res[ids[k]]
+        assert out.status == 0 and out.t == r.t_end
+        m = dg.Mesh.periodic_jittered(r.nx, r.ny, r.x0, r.x1, r.y0, r.y1, r.jitter, r.mesh_seed)
+        o = orc.Oracle.from_mesh(m, r.p, r.prob)
+        ref, n_ref, _ = o.integrate(o.project_ic(orc.ic_array(r.prob, m.period)), r.t_end, 0.3)
+        got = sv.get_state(ids[k], (4, dg.n_modes(r.p), m.num_elem))
+        err = orc.rel_l2(got, ref)
+        print("request %d: %d triangles p=%d t_end %.3f, %d/%d steps, rel L2 vs oracle %.3e"
+              % (k, m.num_elem, r.p, r.t_end, out.steps, n_ref, err), flush=True)
+        assert err <= 1e-5
+        m0, m1 = np.array(out.mass0), np.array(out.mass)
+        assert np.all(np.abs(m1 - m0) <= 2e-6 * np.abs(m0) + 1e-9)
+    sv.close()
