@@ -105,6 +105,7 @@ def lib():
             "dg_solver_info": (I, [VP, ip, dp]),
             "dg_project_ic": (I, [VP]), "dg_set_state": (I, [VP, dp]), "dg_get_state": (I, [VP, dp]),
             "dg_rhs": (I, [VP, dp, dp, D, D]), "dg_rk_step": (I, [VP, D, I]), "dg_cfl_dt": (I, [VP, D, dp]),
+            "dg_rhs_device_noalloc": (I, [VP, VP, VP, D, VP]),
             "dg_integrate": (I, [VP, D, D, I, P(I64), dp]), "dg_run_steps": (I, [VP, I, D, I, I]),
             "dg_audit": (I, [VP, P(AuditOut)]), "dg_eval_vertices": (I, [VP, I, dp]),
             "dg_solver_sync": (I, [VP]), "dg_solver_set_kernel": (I, [VP, I]), "dg_solver_get_kernel": (I, [VP, ip]),
@@ -335,6 +336,13 @@ class Solver:
         k = np.zeros(self.shape)
         _check(lib().dg_rhs(self._h, _dp(u), _dp(k), t, dt))
         return k
+
+    def rhs_device_noalloc(self, u_dev, k_dev, dt=1.0, stream=None):
+        """k = dt * M^-1 R(u) on device buffers (addresses as ints, FP64,
+        reference layout) through the library's fixed scratch: no
+        allocation, enqueued on `stream` (None: the solver's), not waited for."""
+        _check(lib().dg_rhs_device_noalloc(self._h, C.c_void_p(u_dev), C.c_void_p(k_dev), dt,
+                                           C.c_void_p(stream) if stream else None))
 
     def rk_step(self, dt, scheme=DG_RK4):
         _check(lib().dg_rk_step(self._h, dt, scheme))

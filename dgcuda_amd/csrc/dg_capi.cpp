@@ -335,6 +335,12 @@ int dg_rhs(dg_solver* s, const double* u, double* k, double t, double dt) {
     GUARD_BEGIN return ret(s->s->rhs(u, k, dt)); GUARD_END
 }
 
+int dg_rhs_device_noalloc(dg_solver* s, const double* u_dev, double* k_dev, double dt, void* stream) {
+    SOLVER_CHECK(s);
+    if (!u_dev || !k_dev) return fail(DG_ERR_ARG, "null buffer");
+    GUARD_BEGIN return ret(s->s->rhs_device_noalloc(u_dev, k_dev, dt, static_cast<hipStream_t>(stream))); GUARD_END
+}
+
 int dg_rk_step(dg_solver* s, double dt, int scheme) {
     SOLVER_CHECK(s);
     GUARD_BEGIN return ret(s->s->rk_step(dt, scheme)); GUARD_END

@@ -126,6 +126,11 @@ template <typename R>
 hipError_t launch_lambda(const R* u, const StateIx& ix, int K, double b0, const PhysP& ph, Ctrl* ctrl,
                          hipStream_t st);
 hipError_t launch_dt(Ctrl* ctrl, hipStream_t st);
+// reference layout (FP64, caller's order) <-> state array in layout ix
+template <typename R>
+hipError_t launch_ref_to_state(const double* ref, R* dst, const int* perm, const StateIx& ix, int K, hipStream_t st);
+template <typename R>
+hipError_t launch_state_to_ref(const R* src, double* ref, const int* perm, const StateIx& ix, int K, hipStream_t st);
 hipError_t launch_set_dt(Ctrl* ctrl, double dt, int advance, hipStream_t st);
 template <typename R>
 hipError_t launch_project_ic(int p, R* c, const StateIx& ix, const double* V, int K, const IcP& ic, hipStream_t st);

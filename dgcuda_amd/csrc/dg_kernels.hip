@@ -247,6 +247,53 @@ template hipError_t launch_audit<float>(const float*, const StateIx&, const doub
 template hipError_t launch_audit<double>(const double*, const StateIx&, const double*, int, double, double,
                                          double*, int, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// State conversion on the device between the boundary's reference layout
+// (FP64, c[var][mode][elem] in the caller's element order) and a state
+// array of the bulk precision in layout `ix`.  perm[i] (Kp entries) is the
+// caller's element of internal element i; padding elements carry element
+// K-1's id.  One thread per (variable, internal element).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_ref_to_state(const double* __restrict__ ref, R* __restrict__ dst,
+                                                     const int* __restrict__ perm, StateIx ix, int K) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ix.N * ix.Kp) return;
+    const int v = t / ix.Kp, i = t - v * ix.Kp;
+    const size_t src = (size_t)perm[i];
+    // every slot of the layout is written: padding modes hold zeros
+    const int mp = ix.tiled == 0 ? ix.NP : ix.tiled == 2 ? 16 : 4 * tile_groups(ix.NP);
+    for (int m = 0; m < mp; m++)
+        dst[ix.at(v, m, i)] = m < ix.NP ? (R)ref[((size_t)v * ix.NP + m) * K + src] : R(0);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_state_to_ref(const R* __restrict__ src, double* __restrict__ ref,
+                                                     const int* __restrict__ perm, StateIx ix, int K) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ix.N * K) return;
+    const int v = t / K, i = t - v * K;
+    const size_t dst = (size_t)perm[i];
+    for (int m = 0; m < ix.NP; m++) ref[((size_t)v * ix.NP + m) * K + dst] = (double)src[ix.at(v, m, i)];
+}
+
+template <typename R>
+hipError_t launch_ref_to_state(const double* ref, R* dst, const int* perm, const StateIx& ix, int K, hipStream_t st) {
+    const int n = ix.N * ix.Kp;
+    hipLaunchKernelGGL((k_ref_to_state<R>), dim3((n + 255) / 256), dim3(256), 0, st, ref, dst, perm, ix, K);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launch_state_to_ref(const R* src, double* ref, const int* perm, const StateIx& ix, int K, hipStream_t st) {
+    const int n = ix.N * K;
+    hipLaunchKernelGGL((k_state_to_ref<R>), dim3((n + 255) / 256), dim3(256), 0, st, src, ref, perm, ix, K);
+    return hipGetLastError();
+}
+template hipError_t launch_ref_to_state<float>(const double*, float*, const int*, const StateIx&, int, hipStream_t);
+template hipError_t launch_ref_to_state<double>(const double*, double*, const int*, const StateIx&, int, hipStream_t);
+template hipError_t launch_state_to_ref<float>(const float*, double*, const int*, const StateIx&, int, hipStream_t);
+template hipError_t launch_state_to_ref<double>(const double*, double*, const int*, const StateIx&, int, hipStream_t);
+
 // Fill the constant-memory tables of every order from the FP64 host tables.
 template <typename R, int P>
 static void fill_tab(TabP<R, P>& d) {

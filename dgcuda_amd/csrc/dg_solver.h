@@ -29,7 +29,13 @@ struct Status {
 };
 
 Status hip_status(hipError_t e, const char* what);
+// Sets up the current device once per process (tables, the scratch address).
 Status ensure_device_ready();
+
+// Bytes of the device scratch behind the allocation-free right-hand side,
+// and its address on device `dev` (null before the device is set up).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+void* noalloc_scratch(int dev);
 
 // Device memory and streams for solves, kept for reuse within the process:
 // hipFree and hipStreamDestroy synchronise the whole device, which a serving
@@ -104,6 +110,9 @@ class Solver {
     virtual Status set_state(const double* c_ref) = 0;
     virtual Status get_state(double* c_ref) = 0;
     virtual Status rhs(const double* u_ref, double* k_ref, double dt) = 0;
+    // rhs on device buffers (reference layout, FP64) through the device's
+    // scratch: no allocation, enqueued on `on` (null: the solver's stream)
+    virtual Status rhs_device_noalloc(const double* u_dev, double* k_dev, double dt, hipStream_t on) = 0;
     virtual Status rk_step(double dt, int scheme) = 0;
     virtual Status cfl_dt(double cfl, double* dt) = 0;
     virtual Status integrate(double t_end, double cfl, int scheme, int64_t* nsteps) = 0;

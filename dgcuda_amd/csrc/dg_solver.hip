@@ -7,6 +7,7 @@
 // fused launch per stage; dt never leaves the device, and the host only
 // synchronises once per chunk of steps to test for t >= t_end.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstdio>
 #include <tuple>
@@ -30,6 +31,17 @@ Status hip_status(hipError_t e, const char* what) {
         if (_e != hipSuccess) return hip_status(_e, #expr);  \
     } while (0)
 
+// Scratch of the allocation-free right-hand side (rhs_device_noalloc): the
+// stage input and output in the solver's state layout.  A file-scope device
+// array of fixed size; each device that the library sets up holds its own
+// copy until the process ends.  In host code the array's name is not a
+// device address: its address on each device is taken once, at the
+// device's set-up below, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static std::atomic<void*> g_scratch[64];
+
+void* noalloc_scratch(int dev) { return (dev >= 0 && dev < 64) ? g_scratch[dev].load() : nullptr; }
+
 Status ensure_device_ready() {
     static std::mutex mu;
     static bool done[64] = {false};
@@ -41,6 +53,9 @@ Status ensure_device_ready() {
     if (dev < 0 || dev >= 64) return Status::err(DG_ERR_HIP, "device id out of range");
     if (!done[dev]) {
         HIPST(upload_tables());
+        void* ptr = nullptr;
+        HIPST(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch[dev].store(ptr);
         done[dev] = true;
     }
     return Status::ok();
@@ -262,6 +277,8 @@ class SolverT : public Solver {
     R *c = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr;
     R *geo_e = nullptr, *geo_f = nullptr;
     int* nbr = nullptr;
+    int* perm_d = nullptr;  // [Kp]: caller's element of internal element i (padding: element K-1's)
+    int device = -1;
     std::vector<int> h_nbr[3];
     uint16_t* fword = nullptr;
     int* halo = nullptr;
@@ -292,7 +309,8 @@ class SolverT : public Solver {
     ~SolverT() override {
         if (stream) (void)hipStreamSynchronize(stream);  // nothing queued may use the buffers
         if (gexec) (void)hipGraphExecDestroy(gexec);
-        void* bufs[] = {c, acc, ua, ub, geo_e, geo_f, nbr, V, Jd, audit_part, ctrl, fword, halo, face4, metric4, bnd};
+        void* bufs[] = {c,  acc,        ua,   ub,    geo_e, geo_f, nbr,     V,      Jd,
+                        audit_part, ctrl, fword, halo,  face4, metric4, bnd, perm_d};
         for (void* b : bufs) pool_free(b);
         pool_release_stream(stream);
     }
@@ -346,6 +364,7 @@ class SolverT : public Solver {
         min_r = g.min_r;
         Status st = ensure_device_ready();
         if (!st) return st;
+        HIPST(hipGetDevice(&device));
         if (!(st = pool_stream(&stream))) return st;
         six = StateIx{N, NP, Kp, 0};
         const size_t M = state_alloc();
@@ -389,6 +408,10 @@ class SolverT : public Solver {
         HIPST(hipMemcpyAsync(geo_e, he.data(), he.size() * sizeof(R), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(geo_f, hf.data(), hf.size() * sizeof(R), hipMemcpyHostToDevice, stream));
         HIPST(hipMemcpyAsync(nbr, hn.data(), hn.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        std::vector<int> hp(Kp);
+        for (int i = 0; i < Kp; i++) hp[i] = perm[std::min(i, K - 1)];
+        if (!(st = alloc((void**)&perm_d, (size_t)Kp * sizeof(int)))) return st;
+        HIPST(hipMemcpyAsync(perm_d, hp.data(), hp.size() * sizeof(int), hipMemcpyHostToDevice, stream));
         for (int f = 0; f < 3; f++) h_nbr[f] = g.nbr[f];
         // host source of an async copy: lives until the stream is synchronised below
         std::vector<double> hb;
@@ -773,6 +796,32 @@ class SolverT : public Solver {
         lam_valid = false;  // set_dt cleared the accumulated wave speed
         if (!(st = stage(ST_RHS, ub, ub, ua))) return st;
         return download_ref(ua, k_ref);
+    }
+
+    // dg_rhs on the caller's device buffers through the device's scratch:
+    // converts u to the state layout, one RHS stage launch, converts k back;
+    // enqueued on `on` (0: the solver's stream), no allocation, no wait.
+    Status rhs_device_noalloc(const double* u_dev, double* k_dev, double dt, hipStream_t on) override {
+        int cur = -1;
+        HIPST(hipGetDevice(&cur));
+        if (cur != device) return Status::err(DG_ERR_STATE, "the solver's device is not the current device");
+        char* base = static_cast<char*>(noalloc_scratch(device));
+        if (!base) return Status::err(DG_ERR_STATE, "the device's scratch is not set up");
+        const size_t one = (six.elems() * sizeof(R) + 255) / 256 * 256;
+        if (2 * one > NOALLOC_SCRATCH_BYTES)
+            return Status::err(DG_ERR_BUDGET, "this solve needs " + std::to_string(2 * one) +
+                                                  " bytes of scratch; the allocation-free entry point has " +
+                                                  std::to_string(NOALLOC_SCRATCH_BYTES));
+        R* in = reinterpret_cast<R*>(base);
+        R* out = reinterpret_cast<R*>(base + one);
+        hipStream_t sx = on ? on : stream;
+        HIPST(launch_ref_to_state<R>(u_dev, in, perm_d, six, K, sx));
+        HIPST(launch_set_dt(ctrl, dt, 0, sx));
+        lam_valid = false;  // set_dt cleared the accumulated wave speed
+        Status st = stage(ST_RHS, in, in, out, sx);
+        if (!st) return st;
+        HIPST(launch_state_to_ref<R>(out, k_dev, perm_d, six, K, sx));
+        return Status::ok();
     }
 
     Status rk_step(double dt, int scheme) override {

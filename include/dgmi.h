@@ -208,6 +208,20 @@ int dg_get_state(dg_solver* s, double* c_ref);
  * (M = J*I); u and k are host buffers in the reference layout; t is kept for
  * the reference's signature (the boundary states do not depend on it) */
 int dg_rhs(dg_solver* s, const double* u_ref, double* k_ref, double t, double dt);
+/* dg_rhs for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  u_dev and k_dev are the caller's device
+ * buffers of N*n_p*num_elem doubles in the reference layout; k = dt*M^-1 R(u)
+ * as dg_rhs computes it, with the same kernels.  The state in between goes
+ * through a fixed 64 MiB scratch array per device (two arrays of the
+ * solve's state layout: up to about 65,000 triangles for FP64 Euler at
+ * p = 4, 131,000 for FP32); a larger solve returns DG_ERR_BUDGET before any
+ * work.  No allocation and no wait: the work is enqueued on `stream` (a
+ * hipStream_t; NULL = the solver's own stream) and the caller synchronises.
+ * Every call on a device shares that device's scratch, so calls that
+ * overlap on one device (on different streams or threads) race; the
+ * solver's other entry points must not run meanwhile either. */
+int dg_rhs_device_noalloc(dg_solver* s, const double* u_dev, double* k_dev, double dt, void* stream);
 /* one RK step of size dt on the resident state (scheme DG_RK4 / DG_RK2) */
 int dg_rk_step(dg_solver* s, double dt, int scheme);
 /* dt = cfl * min_r / max_K lambda_K / (2p+1) from the resident state */
