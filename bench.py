@@ -158,7 +158,9 @@ def serving_summary():
                        ("stream_1k_to_20k_p1to3_128req_1000per_s", W.serving_small())):
         served, _ = W.run_served(reqs)
         plain, _ = W.run_plain(reqs)
-        r = lambda d: {k: (float("%.4g" % v) if isinstance(v, float) else v) for k, v in d.items()}
+        def r(d):
+            return {k: (float("%.4g" % v) if isinstance(v, float) else r(v) if isinstance(v, dict) else v)
+                    for k, v in d.items()}
         out[name] = dict(served=r(served), plain=r(plain),
                          solves_per_s_ratio=round(served["solves_per_s"] / plain["solves_per_s"], 3))
     return out

@@ -27,7 +27,8 @@ DG_OK = 0
 DG_PHYS_ADVECTION, DG_PHYS_EULER = 1, 4
 DG_FP64, DG_FP32, DG_GEOM_FP32 = 0, 1, 0x100
 DG_RK4, DG_RK2 = 1, 2
-DG_IC_CONSTANT, DG_IC_GAUSSIAN, DG_IC_VORTEX, DG_IC_SUPERSONIC = 0, 1, 2, 3
+DG_IC_CONSTANT, DG_IC_GAUSSIAN, DG_IC_VORTEX, DG_IC_SUPERSONIC, DG_IC_POLY = 0, 1, 2, 3, 4
+DG_FLUX_LLF, DG_FLUX_CENTRAL = 0, 1
 DG_QUIRK_FLOAT_SQRT, DG_QUIRK_FLOAT_GEOM, DG_QUIRK_FLOAT_POW, DG_QUIRK_REFLECT_MIRROR = 1, 2, 4, 8
 DG_GMSH_QUIRKS, DG_GMSH_PERIODIC = 1, 2
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_IO", 3: "DG_ERR_PARSE", 4: "DG_ERR_HIP",
@@ -48,7 +49,8 @@ class ProblemDesc(C.Structure):
         ("gauss_x", C.c_double), ("gauss_y", C.c_double), ("gauss_k", C.c_double),
         ("rho_inf", C.c_double), ("u_inf", C.c_double), ("v_inf", C.c_double), ("p_inf", C.c_double),
         ("vortex_beta", C.c_double * 4), ("vortex_x", C.c_double * 4), ("vortex_y", C.c_double * 4),
-        ("constant", C.c_double * 4), ("mach", C.c_double),
+        ("constant", C.c_double * 4), ("mach", C.c_double), ("wall_x", C.c_double), ("wall_y", C.c_double),
+        ("flux", C.c_int), ("poly_alpha", C.c_int),
     ]
 
 
@@ -91,6 +93,7 @@ def lib():
             "dg_last_error": (C.c_char_p, []), "dg_version": (C.c_char_p, []), "dg_device_count": (I, []),
             "dg_mesh_read_pmsh": (I, [C.c_char_p, P(VP)]),
             "dg_mesh_read_gmsh": (I, [C.c_char_p, I, P(VP)]),
+            "dg_mesh_read_vertex_pmsh": (I, [C.c_char_p, I, P(VP)]),
             "dg_mesh_period": (I, [VP, P(D), P(D)]),
             "dg_mesh_from_arrays": (I, [I, I, dp, ip, dp, ip, D, D, P(VP)]),
             "dg_mesh_periodic_jittered": (I, [I, I, D, D, D, D, D, C.c_uint64, P(VP)]),
@@ -116,6 +119,8 @@ def lib():
             "dg_bandwidth_probe": (I, [I64, I, dp]), "dg_bandwidth_probe_ex": (I, [I64, I, dp, I]),
             "dg_selftest_mfma": (I, [dp]),
             "dg_write_msh": (I, [VP, I, dp, C.c_char_p]),
+            "dg_write_msh_view": (I, [VP, C.c_char_p, I, dp, C.c_char_p]),
+            "dg_eval_vertices_velocity": (I, [VP, dp]), "dg_eval_vertices_pressure": (I, [VP, dp]),
             "dg_l2_error": (I, [VP, P(ProblemDesc), I, dp, dp]),
             "dg_batch_create": (I, [I, P(VP)]), "dg_batch_destroy": (None, [VP]),
             "dg_batch_add": (I, [VP, VP, P(ProblemDesc), I, D, ip]), "dg_batch_project_ic": (I, [VP]),
@@ -212,6 +217,15 @@ class Mesh:
         return cls(h, (px.value, py.value))
 
     @classmethod
+    def read_vertex_pmsh(cls, path, open_kind=-3):
+        """The advection prototype's 6-field .pmsh with its sides built as the
+        prototype's read_mesh does; sides met once get right element
+        open_kind (-3: the problem's exact state, the prototype's boundary)."""
+        h = C.c_void_p()
+        _check(lib().dg_mesh_read_vertex_pmsh(str(path).encode(), open_kind, C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def periodic_jittered(cls, nx, ny, x0, x1, y0, y1, jitter=0.1, seed=0):
         h = C.c_void_p()
         _check(lib().dg_mesh_periodic_jittered(nx, ny, x0, x1, y0, y1, jitter, seed, C.byref(h)))
@@ -251,6 +265,13 @@ class Mesh:
         uv = np.ascontiguousarray(uv, np.float64).reshape(-1)
         _check(lib().dg_write_msh(self._h, var, _dp(uv), str(path).encode()))
 
+    def write_msh_view(self, name, vals, path):
+        """A named gmsh view (the reference CPU driver's output files):
+        vals[K, 3] scalars (ST lines) or vals[K, 3, 2] vectors (VT lines)."""
+        vals = np.ascontiguousarray(vals, np.float64)
+        ncomp = 2 if vals.ndim == 3 else 1
+        _check(lib().dg_write_msh_view(self._h, name.encode(), ncomp, _dp(vals.reshape(-1)), str(path).encode()))
+
     def l2_error(self, prob, p, c):
         """Per-variable L2 error of a state c[N, n_p, K] against the problem's U0."""
         c = np.ascontiguousarray(c, np.float64)
@@ -265,9 +286,13 @@ class Mesh:
 
 
 def problem(physics, ic, gamma=1.4, a=(1.0, 1.0), gauss=(0.0, 0.0, 40.0), background=(1.0, 1.0, 1.0, 1.0),
-            vortices=(), constant=(0.0, 0.0, 0.0, 0.0), quirks=0):
-    """Build a ProblemDesc.  vortices = [(beta, xc, yc), ...]."""
+            vortices=(), constant=(0.0, 0.0, 0.0, 0.0), quirks=0, wall=(0.0, 0.0), flux=DG_FLUX_LLF, alpha=0):
+    """Build a ProblemDesc.  vortices = [(beta, xc, yc), ...]; wall = centre of
+    the reflecting sides' radial normal; flux = DG_FLUX_LLF / DG_FLUX_CENTRAL
+    (advection); alpha = exponent of DG_IC_POLY."""
     d = ProblemDesc()
+    d.wall_x, d.wall_y = wall
+    d.flux, d.poly_alpha = flux, alpha
     d.physics, d.ic, d.quirks, d.n_vortex = physics, ic, quirks, len(vortices)
     d.gamma, d.ax, d.ay = gamma, a[0], a[1]
     d.gauss_x, d.gauss_y, d.gauss_k = gauss
@@ -283,12 +308,28 @@ def advection_gaussian(a=(1.0, 1.0), center=(0.0, 0.0), k=40.0, quirks=0):
     return problem(DG_PHYS_ADVECTION, DG_IC_GAUSSIAN, a=a, gauss=(center[0], center[1], k), quirks=quirks)
 
 
+def advection_poly(alpha, a=(1.0, 1.0), flux=DG_FLUX_CENTRAL, quirks=0):
+    """(x - y)^alpha advected by a, the advection prototype's test problem
+    (src/2d/test_all.cu): stationary for a = (1, 1); central flux by default,
+    as the prototype."""
+    return problem(DG_PHYS_ADVECTION, DG_IC_POLY, a=a, flux=flux, alpha=alpha, quirks=quirks)
+
+
 def euler_vortex(vortices=((5.0, 5.0, 5.0),), gamma=1.4, background=(1.0, 1.0, 1.0, 1.0), quirks=0):
     return problem(DG_PHYS_EULER, DG_IC_VORTEX, gamma=gamma, background=background, vortices=vortices, quirks=quirks)
 
 
 def euler_constant(U, gamma=1.4, quirks=0):
     return problem(DG_PHYS_EULER, DG_IC_CONSTANT, gamma=gamma, constant=U, quirks=quirks)
+
+
+def cylinder(mach=0.38, gamma=1.4, quirks=0):
+    """Flow around a cylinder (euler/cylinder/cylinder.cu): the uniform state
+    rho = gamma, u = mach, v = 0, p = 1 as initial, inflow and outflow state;
+    reflecting sides about the centre (0.5, 0.5).  The reference integrates
+    it with RK2 (DG_RK2)."""
+    U = (gamma, gamma * mach, 0.0, 0.5 * gamma * mach * mach + 1.0 / (gamma - 1.0))
+    return problem(DG_PHYS_EULER, DG_IC_CONSTANT, gamma=gamma, constant=U, quirks=quirks, wall=(0.5, 0.5))
 
 
 def supersonic_vortex(mach=2.25, gamma=1.4, quirks=0):
@@ -370,6 +411,18 @@ class Solver:
         uv = np.zeros((self.num_elem, 3))
         _check(lib().dg_eval_vertices(self._h, var, _dp(uv)))
         return uv
+
+    def eval_vertices_velocity(self):
+        """(u, v) at the three vertices of each element: shape (K, 3, 2)."""
+        uv = np.zeros((self.num_elem, 3, 2))
+        _check(lib().dg_eval_vertices_velocity(self._h, _dp(uv)))
+        return uv
+
+    def eval_vertices_pressure(self):
+        """Pressure at the three vertices of each element: shape (K, 3)."""
+        pv = np.zeros((self.num_elem, 3))
+        _check(lib().dg_eval_vertices_pressure(self._h, _dp(pv)))
+        return pv
 
     def sync(self):
         _check(lib().dg_solver_sync(self._h))

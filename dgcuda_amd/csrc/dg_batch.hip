@@ -124,7 +124,7 @@ __device__ __forceinline__ StageArgs<R> slot_args(const SlotDesc<R>* sd) {
     a.phys.ax = uni(sd->phys.ax);
     a.phys.ay = uni(sd->phys.ay);
     a.phys.quirks = uni(sd->phys.quirks);
-    a.phys.pad = 0;
+    a.phys.flux = uni(sd->phys.flux);
     R* c = uni(sd->c);
     R* ua = uni(sd->ua);
     R* ub = uni(sd->ub);

@@ -96,6 +96,22 @@ int dg_mesh_read_gmsh(const char* path, int flags, dg_mesh** out) {
     GUARD_END
 }
 
+int dg_mesh_read_vertex_pmsh(const char* path, int open_kind, dg_mesh** out) {
+    GUARD_BEGIN
+    if (!path || !out) return fail(DG_ERR_ARG, "null argument");
+    auto* h = new dg_mesh();
+    std::string why = read_vertex_pmsh(path, h->m, open_kind);
+    if (!why.empty()) {
+        delete h;
+        const bool io = why.rfind("cannot open", 0) == 0;
+        const bool arg = why.rfind("open_kind", 0) == 0;
+        return fail(io ? DG_ERR_IO : arg ? DG_ERR_ARG : DG_ERR_PARSE, why);
+    }
+    *out = h;
+    return DG_OK;
+    GUARD_END
+}
+
 int dg_mesh_from_arrays(int K, int S, const double* V, const int* elem_s, const double* sides, const int* side_int,
                         double period_x, double period_y, dg_mesh** out) {
     GUARD_BEGIN
@@ -215,6 +231,33 @@ int dg_write_msh(const dg_mesh* mh, int var, const double* uv, const char* path)
     for (int i = 0; i < m.K; i++)
         fprintf(f, "ST (%lf,%lf,0,%lf,%lf,0,%lf,%lf,0) {%lf,%lf,%lf};\n", m.V1x[i], m.V1y[i], m.V2x[i], m.V2y[i],
                 m.V3x[i], m.V3y[i], uv[3 * (size_t)i], uv[3 * (size_t)i + 1], uv[3 * (size_t)i + 2]);
+    fprintf(f, "};");
+    const bool bad = ferror(f) != 0;
+    fclose(f);
+    if (bad) return fail(DG_ERR_IO, std::string("write error on ") + path);
+    return DG_OK;
+    GUARD_END
+}
+
+int dg_write_msh_view(const dg_mesh* mh, const char* name, int ncomp, const double* vals, const char* path) {
+    GUARD_BEGIN
+    if (!mh || !name || !vals || !path) return fail(DG_ERR_ARG, "null argument");
+    if (ncomp != 1 && ncomp != 2) return fail(DG_ERR_ARG, "ncomp must be 1 (scalar) or 2 (vector)");
+    const Mesh& m = mh->m;
+    FILE* f = fopen(path, "w");
+    if (!f) return fail(DG_ERR_IO, std::string("cannot open ") + path + " for writing");
+    fprintf(f, "View \"%s\" {\n", name);
+    for (int i = 0; i < m.K; i++) {
+        if (ncomp == 1) {
+            const double* v = vals + 3 * (size_t)i;
+            fprintf(f, "ST (%lf,%lf,0,%lf,%lf,0,%lf,%lf,0) {%lf,%lf,%lf};\n", m.V1x[i], m.V1y[i], m.V2x[i],
+                    m.V2y[i], m.V3x[i], m.V3y[i], v[0], v[1], v[2]);
+        } else {
+            const double* v = vals + 6 * (size_t)i;
+            fprintf(f, "VT (%lf,%lf,0,%lf,%lf,0,%lf,%lf,0) {%lf,%lf,0,%lf,%lf,0,%lf,%lf,0};\n", m.V1x[i], m.V1y[i],
+                    m.V2x[i], m.V2y[i], m.V3x[i], m.V3y[i], v[0], v[1], v[2], v[3], v[4], v[5]);
+        }
+    }
     fprintf(f, "};");
     const bool bad = ferror(f) != 0;
     fclose(f);
@@ -379,6 +422,52 @@ int dg_eval_vertices(dg_solver* s, int var, double* uv) {
     SOLVER_CHECK(s);
     if (!uv) return fail(DG_ERR_ARG, "null output");
     GUARD_BEGIN return ret(s->s->eval_vertices(var, uv)); GUARD_END
+}
+
+// vertex values of every variable: [var][K][3]
+static int vertex_values(dg_solver* s, std::vector<double>& w) {
+    const int K = s->s->K;
+    w.assign(4 * 3 * (size_t)K, 0.0);
+    for (int v = 0; v < 4; v++) {
+        const int rc = ret(s->s->eval_vertices(v, w.data() + 3 * (size_t)K * v));
+        if (rc != DG_OK) return rc;
+    }
+    return DG_OK;
+}
+
+int dg_eval_vertices_velocity(dg_solver* s, double* uv) {
+    SOLVER_CHECK(s);
+    if (!uv) return fail(DG_ERR_ARG, "null output");
+    if (s->s->N != 4) return fail(DG_ERR_ARG, "velocity views are for Euler");
+    GUARD_BEGIN
+    std::vector<double> w;
+    const int rc = vertex_values(s, w);
+    if (rc != DG_OK) return rc;
+    const size_t K3 = 3 * (size_t)s->s->K;
+    for (size_t k = 0; k < K3; k++) {
+        uv[2 * k] = w[K3 + k] / w[k];
+        uv[2 * k + 1] = w[2 * K3 + k] / w[k];
+    }
+    return DG_OK;
+    GUARD_END
+}
+
+int dg_eval_vertices_pressure(dg_solver* s, double* pv) {
+    SOLVER_CHECK(s);
+    if (!pv) return fail(DG_ERR_ARG, "null output");
+    if (s->s->N != 4) return fail(DG_ERR_ARG, "pressure views are for Euler");
+    GUARD_BEGIN
+    std::vector<double> w;
+    const int rc = vertex_values(s, w);
+    if (rc != DG_OK) return rc;
+    const size_t K3 = 3 * (size_t)s->s->K;
+    const double g = s->s->prob.gamma;
+    for (size_t k = 0; k < K3; k++) {
+        const double rho = w[k], u = w[K3 + k] / rho, v = w[2 * K3 + k] / rho, E = w[3 * K3 + k];
+        pv[k] = (g - 1.) * (E - (u * u + v * v) / 2. * rho);
+    }
+    return DG_OK;
+    GUARD_END
 }
 
 int dg_solver_sync(dg_solver* s) {

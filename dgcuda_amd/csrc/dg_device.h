@@ -130,7 +130,7 @@ struct PhysP {
     double gamma;
     double ax, ay;
     int quirks;      // DG_QUIRK_* bits
-    int pad;
+    int flux;        // advection: DG_FLUX_LLF (0) or DG_FLUX_CENTRAL (1, lambda = 0)
 };
 
 // Packed neighbour word per (element, face): bits 0..27 neighbour element,
@@ -147,10 +147,15 @@ __host__ __device__ inline int nb_pack_boundary(int bfi, int kind) {
 enum { BC_REFLECTING = 1, BC_OUTFLOW = 2, BC_INFLOW = 3 };
 // Highest order whose lane kernel carries the boundary path.  The first
 // run of the p = 4 FP64 instance with it faulted the GPU (memory aperture
-// violation) and the cause is not found, so p >= 4 keeps the kernel without
-// it; boundary meshes at p = 4 run on the matrix-core kernels' boundary
-// variant (Euler), and p = 5 is refused.
-constexpr int VALU_BOUNDARY_MAX_P = 3;
+// violation) and the cause is not found (a host build of the body checks
+// every global access of that instance in bounds on the shipped boundary
+// meshes, tests/test_lane_kernel_host.py), so p >= 4 keeps the kernel
+// without it; boundary meshes at p = 4 run on the matrix-core kernels'
+// boundary variant (Euler), and p = 5 is refused.  The host build raises it.
+#ifndef DGMI_VALU_BOUNDARY_MAX_P
+#define DGMI_VALU_BOUNDARY_MAX_P 3
+#endif
+constexpr int VALU_BOUNDARY_MAX_P = DGMI_VALU_BOUNDARY_MAX_P;
 
 enum StageKind {
     ST_RHS = 0,   // out = dt * M^-1 R(u)
@@ -234,7 +239,9 @@ __device__ __forceinline__ int tile_of_block(int b, int nblocks, int remap) {
 // to the compiler, which then issues FLAT accesses with 64-bit per-lane
 // addresses.  Reading and writing through a global-address-space view
 // gives scalar-base global loads whatever the pointer's origin.
+#ifndef DG_GLOBAL  // a host build of these headers (tests/host_check) defines it empty
 #define DG_GLOBAL __attribute__((address_space(1)))
+#endif
 template <typename T>
 __device__ __forceinline__ DG_GLOBAL T* gp(T* p) {
     return (DG_GLOBAL T*)p;
@@ -301,7 +308,9 @@ __device__ __forceinline__ R sound_speed(const PhysP& ph, const R* U) {
 template <int N, typename R>
 __device__ __forceinline__ R llf_speed(const PhysP& ph, const R* UL, const R* UR, R nx, R ny) {
     if constexpr (N == 1) {
-        return fabs((R)ph.ax * nx + (R)ph.ay * ny);
+        // central flux: no dissipation term (the advection prototype's
+        // riemann = (u_L + u_R)/2, 2dadvec_kernels.cu:162-164)
+        return ph.flux ? R(0) : fabs((R)ph.ax * nx + (R)ph.ay * ny);
     } else {
         if constexpr (sizeof(R) == 8) {
             R cl = sound_speed<R>(ph, UL), cr = sound_speed<R>(ph, UR);

@@ -7,7 +7,7 @@ namespace dgmi {
 // initial-condition parameters passed by value to the projection kernel;
 // also the state of inflow and outflow boundary sides
 struct IcP {
-    int kind;   // 0 constant, 1 Gaussian pulse, 2 vortex superposition, 3 supersonic vortex
+    int kind;   // 0 constant, 1 Gaussian pulse, 2 vortex superposition, 3 supersonic vortex, 4 (x - y)^alpha
     int nv;
     double gamma;
     double gx, gy, gk;
@@ -17,7 +17,7 @@ struct IcP {
     double constant[4];
     double mach;  // supersonic vortex: Mach number at r = 1
     int fpow;     // supersonic vortex: powers rounded through float
-    int pad;
+    int alpha;    // (x - y)^alpha
 };
 
 // Initial condition U0(x, y) in FP64 (conserv_kernels.cu:167-187,
@@ -38,6 +38,12 @@ __host__ __device__ inline void ic_point(const IcP& ic, double x, double y, doub
     if (ic.kind == 1) {
         double dx = wrap_d(x - ic.gx, ic.Lx), dy = wrap_d(y - ic.gy, ic.Ly);
         U[0] = exp(-ic.gk * (dx * dx + dy * dy));
+        return;
+    }
+    if (ic.kind == 4) {
+        // the advection prototype's test function (test_all.cu:21-51)
+        const double u = pow(x - y, (double)ic.alpha);
+        for (int n = 0; n < 4; n++) U[n] = u;
         return;
     }
     if (ic.kind == 3) {

@@ -57,6 +57,77 @@ std::string read_pmsh(const char* path, Mesh& m) {
     return validate(m);
 }
 
+// .pmsh, advection-prototype dialect (src/2d/2dadvec/2dadvec.cu:114-252):
+// num_elem, then "V1x V1y V2x V2y V3x V3y" per element.  Sides are built from
+// the elements in file order: a clockwise element first has V1 and V2
+// swapped; each face (V1V2, V2V3, V3V1) is looked up among the sides made so
+// far by its two end points' exact coordinates (either orientation); found,
+// this element becomes the side's right element; not found, a new side is
+// made with this element on its left and the face's end points in this
+// element's orientation.  Sides met once get right element `open_kind`.
+std::string read_vertex_pmsh(const char* path, Mesh& m, int open_kind) {
+    if (open_kind > -1 || open_kind < -3) return "open_kind must be -1, -2 or -3";
+    FILE* f = fopen(path, "r");
+    if (!f) return std::string("cannot open mesh file ") + path;
+    char line[1024];
+    int K = 0;
+    if (!fgets(line, sizeof line, f) || sscanf(line, "%d", &K) != 1 || K <= 0) {
+        fclose(f);
+        return "bad element count line";
+    }
+    m = Mesh();
+    m.K = K;
+    m.V1x.resize(K); m.V1y.resize(K); m.V2x.resize(K); m.V2y.resize(K); m.V3x.resize(K); m.V3y.resize(K);
+    m.elem_s1.resize(K); m.elem_s2.resize(K); m.elem_s3.resize(K);
+    for (int i = 0; i < K; i++) {
+        if (!fgets(line, sizeof line, f) ||
+            sscanf(line, "%lf %lf %lf %lf %lf %lf", &m.V1x[i], &m.V1y[i], &m.V2x[i], &m.V2y[i], &m.V3x[i],
+                   &m.V3y[i]) != 6) {
+            fclose(f);
+            return "element line " + std::to_string(i) + ": expected 6 coordinates";
+        }
+    }
+    fclose(f);
+    // a side's key: its end points, the smaller (x, y) first
+    using Pt = std::pair<double, double>;
+    std::map<std::pair<Pt, Pt>, int> found;
+    auto key = [](Pt a, Pt b) { return a < b ? std::make_pair(a, b) : std::make_pair(b, a); };
+    for (int i = 0; i < K; i++) {
+        const double J = (m.V2x[i] - m.V1x[i]) * (m.V3y[i] - m.V1y[i]) - (m.V3x[i] - m.V1x[i]) * (m.V2y[i] - m.V1y[i]);
+        if (J < 0) {
+            std::swap(m.V1x[i], m.V2x[i]);
+            std::swap(m.V1y[i], m.V2y[i]);
+        }
+        const Pt P[3] = {{m.V1x[i], m.V1y[i]}, {m.V2x[i], m.V2y[i]}, {m.V3x[i], m.V3y[i]}};
+        int* es[3] = {&m.elem_s1[i], &m.elem_s2[i], &m.elem_s3[i]};
+        for (int fc = 0; fc < 3; fc++) {
+            const Pt a = P[fc], b = P[(fc + 1) % 3];
+            auto it = found.find(key(a, b));
+            if (it != found.end() && m.right_elem[it->second] == open_kind) {
+                const int s = it->second;
+                m.right_elem[s] = i;
+                m.right_side[s] = fc;
+                *es[fc] = s;
+                continue;
+            }
+            if (it != found.end()) return "side met by more than two elements (element " + std::to_string(i) + ")";
+            const int s = m.S++;
+            found[key(a, b)] = s;
+            m.sx1.push_back(a.first);
+            m.sy1.push_back(a.second);
+            m.sx2.push_back(b.first);
+            m.sy2.push_back(b.second);
+            m.left_elem.push_back(i);
+            m.right_elem.push_back(open_kind);
+            m.left_side.push_back(fc);
+            m.right_side.push_back(0);
+            *es[fc] = s;
+        }
+    }
+    m.period_x = m.period_y = 0;
+    return validate(m);
+}
+
 // Periodic pairing of the open, untagged sides on the edges of the nodes'
 // bounding box: a side on x = xmin pairs with the side on x = xmax whose end
 // points are its own moved by the box width, and likewise in y.  The side met

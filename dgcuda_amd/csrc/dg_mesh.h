@@ -51,6 +51,11 @@ std::string read_pmsh(const char* path, Mesh& m);
 std::string read_gmsh(const char* path, Mesh& m, bool genmesh_quirks, bool periodic);
 std::string write_pmsh(const char* path, const Mesh& m);
 
+// The advection prototype's 6-field .pmsh (V1..V3 per element) with its sides
+// built as the prototype's read_mesh does (src/2d/2dadvec/2dadvec.cu:114-252);
+// sides met by one element get right element `open_kind` (-1, -2 or -3).
+std::string read_vertex_pmsh(const char* path, Mesh& m, int open_kind);
+
 // n_x * n_y squares on [x0,x1] x [y0,y1], periodic in both directions, each
 // split along its (i,j)-(i+1,j+1) diagonal into two counter-clockwise
 // triangles.  Lattice vertices off the periodic boundary are moved by

@@ -161,6 +161,7 @@ PhysP make_phys(const dg_problem_desc& d) {
     ph.ax = d.ax;
     ph.ay = d.ay;
     ph.quirks = d.quirks;
+    ph.flux = d.flux;
     return ph;
 }
 
@@ -186,20 +187,23 @@ IcP make_ic(const dg_problem_desc& d, double Lx, double Ly) {
     }
     ic.mach = d.mach;
     ic.fpow = (d.quirks & DG_QUIRK_FLOAT_POW) ? 1 : 0;
+    ic.alpha = d.poly_alpha;
     return ic;
 }
 
 // Per boundary side and face point, 4 FP64 values for the lane kernel
 // (boundary_ghost): inflow and outflow (euler.cu:177-241) take the problem's
-// state U0 at the point; reflecting (supersonic.cu:64-108) the unit radial
-// direction there, turned to the side's normal, as the wall normal.  The
+// state U0 at the point; reflecting (supersonic.cu:64-108,
+// cylinder.cu:58-101) the unit direction from the problem's wall centre to
+// the point, turned to the side's normal, as the wall normal.  The
 // point is the left element's face point j mapped by the reference's
 // formula x = x2 r + x3 s + x1 (1 - r - s): face 0 (0.5 + 0.5 r_j, 0), face
 // 1 ((1 - r_j)/2, (1 + r_j)/2), face 2 (0, 0.5 + 0.5 r_{n-1-j}); the
 // reflecting side maps face 1 as ((1 + r_j)/2, (1 - r_j)/2) under
 // DG_QUIRK_REFLECT_MIRROR, as the reference does, and takes the radial
 // length through a float square root under DG_QUIRK_FLOAT_SQRT.
-std::vector<double> boundary_data(const ElemGeometry& g, const Tables& T, const IcP& ic, int quirks) {
+std::vector<double> boundary_data(const ElemGeometry& g, const Tables& T, const IcP& ic, int quirks, double wall_x,
+                                  double wall_y) {
     const int n1 = T.n_q1;
     std::vector<double> out((size_t)g.bfaces.size() * n1 * 4, 0.0);
     for (size_t b = 0; b < g.bfaces.size(); b++) {
@@ -221,9 +225,10 @@ std::vector<double> boundary_data(const ElemGeometry& g, const Tables& T, const 
             const double y = v2y * r1 + v3y * r2 + v1y * (1 - r1 - r2);
             double* o = &out[(b * n1 + j) * 4];
             if (kind == BC_REFLECTING) {
-                const double d2 = x * x + y * y;
+                const double xw = x - wall_x, yw = y - wall_y;
+                const double d2 = xw * xw + yw * yw;
                 const double dot = (quirks & DG_QUIRK_FLOAT_SQRT) ? (double)std::sqrt((float)d2) : std::sqrt(d2);
-                double Nx = x / dot, Ny = y / dot;
+                double Nx = xw / dot, Ny = yw / dot;
                 if (Nx * g.fnx[f][e] + Ny * g.fny[f][e] < 0) {
                     Nx *= -1;
                     Ny *= -1;
@@ -243,7 +248,12 @@ Status check_problem(const dg_problem_desc& d, int p, int precision) {
     if (precision != DG_FP64 && precision != DG_FP32) return Status::err(DG_ERR_ARG, "precision must be DG_FP64 or DG_FP32");
     if (d.physics != DG_PHYS_ADVECTION && d.physics != DG_PHYS_EULER)
         return Status::err(DG_ERR_ARG, "physics must be DG_PHYS_ADVECTION or DG_PHYS_EULER");
-    if (d.ic < 0 || d.ic > 3) return Status::err(DG_ERR_ARG, "unknown initial condition");
+    if (d.ic < 0 || d.ic > 4) return Status::err(DG_ERR_ARG, "unknown initial condition");
+    if (d.ic == DG_IC_POLY && (d.poly_alpha < 0 || d.poly_alpha > 8))
+        return Status::err(DG_ERR_ARG, "poly_alpha must be in 0..8");
+    if (d.flux != DG_FLUX_LLF && d.flux != DG_FLUX_CENTRAL) return Status::err(DG_ERR_ARG, "unknown flux");
+    if (d.flux == DG_FLUX_CENTRAL && d.physics != DG_PHYS_ADVECTION)
+        return Status::err(DG_ERR_ARG, "the central flux is for advection");
     if (d.ic == DG_IC_SUPERSONIC && (d.physics != DG_PHYS_EULER || !(d.mach > 0.0)))
         return Status::err(DG_ERR_ARG, "supersonic vortex needs Euler physics and a positive Mach number");
     if (d.physics == DG_PHYS_ADVECTION && d.ic == DG_IC_VORTEX)
@@ -254,6 +264,7 @@ Status check_problem(const dg_problem_desc& d, int p, int precision) {
         return Status::err(DG_ERR_ARG, "n_vortex must be in 1..4");
     if (d.physics == DG_PHYS_EULER && !(d.gamma > 1.0)) return Status::err(DG_ERR_ARG, "gamma must exceed 1");
     if (d.quirks & ~15) return Status::err(DG_ERR_ARG, "unknown quirk bits");
+    if (!std::isfinite(d.wall_x) || !std::isfinite(d.wall_y)) return Status::err(DG_ERR_ARG, "wall centre not finite");
     return Status::ok();
 }
 
@@ -416,7 +427,7 @@ class SolverT : public Solver {
         // host source of an async copy: lives until the stream is synchronised below
         std::vector<double> hb;
         if (has_boundary) {
-            hb = boundary_data(g, build_tables(p), ic, pr.quirks);
+            hb = boundary_data(g, build_tables(p), ic, pr.quirks, pr.wall_x, pr.wall_y);
             if (!(st = alloc((void**)&bnd, hb.size() * sizeof(double)))) return st;
             HIPST(hipMemcpyAsync(bnd, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         }

@@ -148,9 +148,16 @@ def run_served(reqs, precision=DG_FP32, max_slots=64, hbm_budget=64 << 30, steps
 
 def run_plain(reqs, precision=DG_FP32, keep=()):
     """One request at a time on the single-solve path, in arrival order:
-    build the mesh and solver, project, integrate to t_end, audit, free."""
+    build the mesh and solver, project, integrate to t_end, audit, free.
+
+    Also the ceiling of any batching of this stream: each request's work
+    (DOF*stage = K n_p N 4 steps) and the time its integration took alone
+    give per order p the best rate a single solve reached (that of the
+    largest solves, which fill the GPU); the stream's work at those rates is
+    the least time any server can take with these kernels, and the plain
+    makespan over it the largest served/plain ratio possible."""
     t0 = time.perf_counter() + 0.05
-    lat, states = [], {}
+    lat, states, runs = [], {}, []
     for k, r in enumerate(reqs):
         now = time.perf_counter()
         if now < t0 + r.arrival_s:
@@ -158,13 +165,24 @@ def run_plain(reqs, precision=DG_FP32, keep=()):
         m = Mesh.periodic_jittered(r.nx, r.ny, r.x0, r.x1, r.y0, r.y1, r.jitter, r.mesh_seed)
         s = Solver(m, r.prob, r.p, precision)
         s.project_ic()
-        s.integrate(r.t_end, 0.3)
+        s.sync()
+        ti = time.perf_counter()
+        n, _ = s.integrate(r.t_end, 0.3)
+        ti = time.perf_counter() - ti
         s.audit()
         if k in keep:
             states[k] = s.get_state()
+        runs.append((r.p, m.num_elem * n_modes(r.p) * r.prob.physics * 4 * n, ti))
         s.close()
         lat.append(time.perf_counter() - (t0 + r.arrival_s))
     makespan = time.perf_counter() - t0
     out = dict(solves=len(reqs), makespan_s=makespan, solves_per_s=len(reqs) / makespan)
     out.update({"latency_" + k: v for k, v in _latency_stats(lat).items()})
+    best = {}
+    for p, work, ti in runs:
+        best[p] = max(best.get(p, 0.0), work / ti)
+    ideal = sum(work / best[p] for p, work, _ in runs)
+    out.update(work_dof_stage=float(sum(w for _, w, _ in runs)), integrate_s=float(sum(t for _, _, t in runs)),
+               best_rate_by_p={str(p): best[p] for p in sorted(best)}, ideal_makespan_s=ideal,
+               ceiling_ratio=makespan / ideal)
     return out, states

@@ -68,8 +68,20 @@ enum { DG_FP64 = 0, DG_FP32 = 1, DG_GEOM_FP32 = 0x100 };
 enum { DG_RK4 = 1, DG_RK2 = 2 };
 /* initial conditions; DG_IC_SUPERSONIC is the supersonic vortex between two
  * arcs (euler/supersonic/supersonic.cu:28-38), also the state its inflow and
- * outflow boundaries impose */
-enum { DG_IC_CONSTANT = 0, DG_IC_GAUSSIAN = 1, DG_IC_VORTEX = 2, DG_IC_SUPERSONIC = 3 };
+ * outflow boundaries impose.  The cylinder problem (euler/cylinder/cylinder.cu:
+ * 26-111) is DG_IC_CONSTANT with the state (gamma, gamma M, 0,
+ * gamma M^2/2 + 1/(gamma - 1)), which its inflow and outflow sides also
+ * impose, and the wall centre (0.5, 0.5); the reference runs it with RK2. */
+enum { DG_IC_CONSTANT = 0, DG_IC_GAUSSIAN = 1, DG_IC_VORTEX = 2, DG_IC_SUPERSONIC = 3, DG_IC_POLY = 4 };
+/* DG_IC_POLY: u = (x - y)^poly_alpha in every variable, the advection
+ * prototype's projection and time-step test function (src/2d/test_all.cu:
+ * 21-51, 275-498): with a = (1, 1) it is a stationary solution, and its
+ * value is also the state of inflow / outflow (exact Dirichlet) sides. */
+/* numerical flux of advection: DG_FLUX_LLF the local Lax-Friedrichs flux
+ * (lambda = |a.n|, upwind); DG_FLUX_CENTRAL the advection prototype's
+ * central flux F(s).n with s = (u_L + u_R)/2 (2dadvec_kernels.cu:162-164,
+ * 533-570), evaluated as the LLF form with lambda = 0.  Euler takes LLF. */
+enum { DG_FLUX_LLF = 0, DG_FLUX_CENTRAL = 1 };
 /* optional reproduction of the reference's float roundings and quirks:
  *  FLOAT_SQRT:     sound speed, cell wave speed and the reflecting wall's
  *                  radial length through a float square root (euler.cu:80,
@@ -100,6 +112,11 @@ typedef struct {
     double vortex_beta[DG_MAX_VORTEX], vortex_x[DG_MAX_VORTEX], vortex_y[DG_MAX_VORTEX];
     double constant[4]; /* DG_IC_CONSTANT: conserved state */
     double mach;        /* DG_IC_SUPERSONIC: Mach number at the inner arc r = 1 (the reference's 2.25) */
+    double wall_x, wall_y; /* reflecting sides: the wall normal at a face point is the direction from
+                              this centre to the point (the supersonic vortex's arcs: (0, 0),
+                              supersonic.cu:92-99; the cylinder: (0.5, 0.5), cylinder.cu:84-90) */
+    int flux;           /* DG_FLUX_LLF or DG_FLUX_CENTRAL (advection only) */
+    int poly_alpha;     /* DG_IC_POLY: the exponent alpha (0..8) */
 } dg_problem_desc;
 
 typedef struct {
@@ -142,6 +159,17 @@ int dg_mesh_read_pmsh(const char* path, dg_mesh** out);
  * no tag and no partner: DG_ERR_PARSE. */
 enum { DG_GMSH_QUIRKS = 1, DG_GMSH_PERIODIC = 2 };
 int dg_mesh_read_gmsh(const char* path, int flags, dg_mesh** out);
+/* reads the advection prototype's .pmsh dialect (a line with num_elem, then
+ * V1x V1y V2x V2y V3x V3y per element) and builds its sides as the
+ * prototype's read_mesh does (src/2d/2dadvec/2dadvec.cu:114-252): a clockwise
+ * element has V1 and V2 swapped; faces V1V2, V2V3, V3V1 are matched with the
+ * sides met so far by their end points' coordinates; an unmatched face makes
+ * a new side with this element on its left; a matched one gets this element
+ * on its right.  Faces met once are boundary sides; they get right element
+ * `open_kind` (-1 reflecting, -2 outflow, -3 inflow): the prototype's
+ * boundary sides take the exact solution (boundary_exact,
+ * 2dadvec_kernels.cu:184), i.e. -3 with a stationary initial condition. */
+int dg_mesh_read_vertex_pmsh(const char* path, int open_kind, dg_mesh** out);
 /* the mesh's periodic extent in x and y (0 for a direction that does not
  * wrap); the initial conditions measure distances to the nearest image over
  * it.  No counterpart in the reference, whose meshes are not periodic. */
@@ -170,6 +198,15 @@ void dg_mesh_destroy(dg_mesh* m);
  * one "ST (x1,y1,0,x2,y2,0,x3,y3,0) {u1,u2,u3};" line per element in the
  * mesh's order (%lf), "};".  uv[3*K] as dg_eval_vertices returns it. */
 int dg_write_msh(const dg_mesh* m, int var, const double* uv, const char* path);
+/* a named gmsh view as the reference's CPU Euler driver writes its output
+ * (euler/cpu/main.c:201-263): "View \"<name>\" {", then per element in the
+ * mesh's order a scalar line "ST (x1,y1,0,x2,y2,0,x3,y3,0) {f1,f2,f3};"
+ * (ncomp = 1, vals[3*K]) or a vector line
+ * "VT (x1,y1,0,...) {u1,v1,0,u2,v2,0,u3,v3,0};" (ncomp = 2, vals[6*K] =
+ * u1,v1,u2,v2,u3,v3 per element), all numbers %lf, then "};".  The
+ * reference's names are "Density ", "u ", "E " and, for its pressure file,
+ * "E " again. */
+int dg_write_msh_view(const dg_mesh* m, const char* name, int ncomp, const double* vals, const char* path);
 
 /* L2 error of a state (reference layout, FP64, the mesh's element order)
  * against the problem's initial condition U0, per variable:
@@ -235,6 +272,14 @@ int dg_run_steps(dg_solver* s, int nsteps, double cfl, int scheme, int use_graph
 int dg_audit(dg_solver* s, dg_audit_out* out);
 /* uv[3*num_elem]: variable var at the three vertices of each element (eval_u) */
 int dg_eval_vertices(dg_solver* s, int var, double* uv);
+/* Euler: the velocity at the three vertices of each element, u = (rho u)/rho
+ * and v = (rho v)/rho from the vertex values (eval_u_velocity,
+ * euler/cpu/euler_kernels.c:1131-1172); uv[6*K] = u1,v1,u2,v2,u3,v3 per element */
+int dg_eval_vertices_velocity(dg_solver* s, double* uv);
+/* Euler: the pressure at the three vertices of each element,
+ * (gamma - 1)(E - (u^2 + v^2)/2 rho) from the vertex values (eval_p and
+ * pressure, euler/cpu/euler_kernels.c:124-140, 1177-1240); pv[3*K] */
+int dg_eval_vertices_pressure(dg_solver* s, double* pv);
 int dg_solver_sync(dg_solver* s);
 /* kernel selection for the stage kernels: 0 = automatic (timed per
  * physics, order, precision and size), 1 = one element per lane (VALU),

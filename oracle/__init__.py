@@ -51,7 +51,8 @@ def lib():
             "or_tables": (None, [VP] + [dp] * 10),
             "or_geometry": (D, [VP, dp, dp]),
             "or_project_ic": (None, [VP, dp, dp]),
-            "or_set_bc_ic": (None, [VP, dp, I]),
+            "or_set_bc_ic": (None, [VP, dp, I]), "or_set_wall": (None, [VP, D, D]),
+            "or_set_flux": (None, [VP, I]),
             "or_ic_point": (None, [dp, I, D, D, D, dp]),
             "or_rhs": (I, [VP, dp, dp, D]),
             "or_rhs_parts": (I, [VP, dp, dp, dp, dp]),
@@ -111,6 +112,8 @@ def ic_array(prob, period=(0.0, 0.0)):
         return np.array([1.0, prob.gauss_x, prob.gauss_y, prob.gauss_k, period[0], period[1]])
     if prob.ic == 3:  # supersonic vortex: {4, mach, float_pow}
         return np.array([4.0, prob.mach, 1.0 if prob.quirks & 4 else 0.0])
+    if prob.ic == 4:  # (x - y)^alpha: {3, alpha}
+        return ic_poly(prob.poly_alpha)
     a = [2.0, float(prob.n_vortex), prob.rho_inf, prob.u_inf, prob.v_inf, prob.p_inf, period[0], period[1]]
     for k in range(prob.n_vortex):
         a += [prob.vortex_beta[k], prob.vortex_x[k], prob.vortex_y[k]]
@@ -140,9 +143,20 @@ class Oracle:
     def from_mesh(cls, mesh, p, prob, hoist=1):
         V, es, sd, si = mesh.arrays()
         o = cls(V, es, sd, si, p, prob.physics, prob.gamma, (prob.ax, prob.ay), prob.quirks, hoist)
-        if (np.asarray(si).reshape(4, -1)[1] < 0).any():  # boundary sides: inflow / outflow states
+        if prob.flux:
+            o.set_flux(prob.flux)
+        if (np.asarray(si).reshape(4, -1)[1] < 0).any():  # boundary sides: inflow / outflow states, wall centre
             o.set_bc_ic(ic_array(prob, mesh.period))
+            o.set_wall(prob.wall_x, prob.wall_y)
         return o
+
+    def set_flux(self, flux):
+        """Advection flux: 0 local Lax-Friedrichs, 1 central (lambda = 0)."""
+        lib().or_set_flux(self._h, flux)
+
+    def set_wall(self, x, y):
+        """Centre of the reflecting sides' radial wall normal."""
+        lib().or_set_wall(self._h, x, y)
 
     def set_bc_ic(self, ic):
         """IC parameter array (as for project_ic) of the inflow / outflow states."""

@@ -112,6 +112,8 @@ struct Ctx {
     int physics = 0, quirks = 0, hoist = 1;
     bool has_boundary = false;  // boundary sides: the ghost states need bc_ic
     std::vector<double> bc_ic;  // IC parameters of the inflow / outflow states
+    double wall_x = 0, wall_y = 0;  // centre of the reflecting sides' radial normal
+    int central = 0;                // advection: the prototype's central flux
     double gamma = 1.4, ax = 1.0, ay = 1.0;
     std::vector<double> V1x, V1y, V2x, V2y, V3x, V3y;
     std::vector<int> es1, es2, es3, left_elem, right_elem, lsn, rsn;
@@ -251,7 +253,10 @@ inline void eval_flux(const Ctx& c, const double* U, double* fx, double* fy) {
 }
 
 inline double eval_lambda(const Ctx& c, const double* UL, const double* UR, double nx, double ny) {
-    if (c.physics == PHYS_ADVECTION) return fabs(c.ax * nx + c.ay * ny);
+    /* advection: upwind |a.n|; the prototype's central flux
+     * (2dadvec_kernels.cu:162-164, riemann = (u_L + u_R)/2) has no
+     * dissipation term */
+    if (c.physics == PHYS_ADVECTION) return c.central ? 0. : fabs(c.ax * nx + c.ay * ny);
     double cl = eval_c(c, UL), cr = eval_c(c, UR);
     double ul = UL[1] / UL[0], vl = UL[2] / UL[0];
     double ur = UR[1] / UR[0], vr = UR[2] / UR[0];
@@ -300,6 +305,10 @@ void ghost_state(const Ctx& c, int code, int l, int ls, int j, double nx, double
     double x, y;
     if (code == BC_REFLECTING) {
         face_point(c, l, ls, j, (c.quirks & QUIRK_REFLECT_MIRROR) != 0, &x, &y);
+        /* the wall centre: the origin for the supersonic vortex, (0.5, 0.5)
+         * for the cylinder (cylinder.cu:84-90) */
+        x -= c.wall_x;
+        y -= c.wall_y;
         UR[0] = UL[0];
         if (c.N == 1) return;
         UR[3] = UL[3];
@@ -614,6 +623,15 @@ void or_destroy(void* h) { delete (Ctx*)h; }
 
 /* IC parameters (the or_project_ic array) of the inflow / outflow states */
 void or_set_bc_ic(void* h, const double* ic, int n) { ((Ctx*)h)->bc_ic.assign(ic, ic + n); }
+
+/* advection flux: 0 local Lax-Friedrichs, 1 the prototype's central flux */
+void or_set_flux(void* h, int central) { ((Ctx*)h)->central = central; }
+
+/* centre of the reflecting sides' radial wall normal */
+void or_set_wall(void* h, double x, double y) {
+    ((Ctx*)h)->wall_x = x;
+    ((Ctx*)h)->wall_y = y;
+}
 
 void or_set_hoist(void* h, int hoist) { ((Ctx*)h)->hoist = hoist; }
 

@@ -19,6 +19,16 @@ fixtures it writes are committed, so the tests never read the reference.
   gmsh/             .msh / .pmsh pairs the reference's genmesh.py made:
                     sv1, sv1refined (supersonic/mesh), cylinder, cylinder1
                     (cylinder/mesh)
+  cpu_output/       the gmsh views the reference's CPU Euler driver wrote on
+                    sv1 (src/2d/conservation/euler/cpu/output/uniform_rho.out,
+                    uniform_u.out, uniform_E.out, p.out; main.c:201-263)
+  t1/ t10/ t100/    the reference's time-step golden vectors
+                    src/2d/tests/t{1,10,100}/alpha{a}_{mesh}{p}.out (central
+                    flux, exact boundary values, RK4; test_all.cu:275-498,
+                    614-705) for the meshes canonical and box.  The uniform
+                    mesh's dumps are left out: at p >= 1 they grow without
+                    bound (dt = 0.01 exceeds that mesh's stable step, values
+                    up to 1e244), so they pin nothing.
 """
 import json
 import math
@@ -92,6 +102,17 @@ def main():
         for ext in (".msh", ".pmsh"):
             shutil.copyfile(os.path.join(src2d, "conservation", "euler", sub, "mesh", name + ext),
                             os.path.join(gm, name + ext))
+    co = os.path.join(OUT, "cpu_output")
+    os.makedirs(co, exist_ok=True)
+    for name in ("uniform_rho.out", "uniform_u.out", "uniform_E.out", "p.out"):
+        shutil.copyfile(os.path.join(src2d, "conservation", "euler", "cpu", "output", name), os.path.join(co, name))
+    for steps in (1, 10, 100):
+        d = os.path.join(OUT, "t%d" % steps)
+        os.makedirs(d, exist_ok=True)
+        src = os.path.join(src2d, "tests", "t%d" % steps)
+        for name in sorted(os.listdir(src)):
+            if re.fullmatch(r"alpha\d_(canonical|box)\d\.out", name):
+                shutil.copyfile(os.path.join(src, name), os.path.join(d, name))
     print("fixtures written to", OUT)
 
 

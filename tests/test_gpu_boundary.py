@@ -112,3 +112,55 @@ def test_boundary_mesh_kernel_choice():
         b.add(m, prob, 2, 0.1)
     assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
     b.close()
+
+
+def cylinder_mesh(name):
+    import os
+    from helpers import GOLD_DIR
+
+    return dg.Mesh.read_pmsh(os.path.join(GOLD_DIR, "gmsh", name + ".pmsh"))
+
+
+@pytest.mark.parametrize("quirks", [0, 15])
+@pytest.mark.parametrize("prec", [dg.DG_FP64, dg.DG_FP32])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_cylinder_rhs_parity(p, prec, quirks):
+    """The cylinder problem (euler/cylinder/cylinder.cu): reflecting walls
+    about (0.5, 0.5) and the uniform Mach 0.38 far field, on the reference's
+    cylinder1 mesh; every kernel that takes boundary sides at this order."""
+    print("cylinder rhs p=%d prec=%d quirks=%d" % (p, prec, quirks), flush=True)
+    m = cylinder_mesh("cylinder1")
+    prob = dg.cylinder(quirks=quirks)
+    o = orc.Oracle.from_mesh(m, p, prob)
+    u = perturbed_state(o, prob, m.period, seed=p, amp=1e-2)
+    ref = o.rhs(u, dt=0.01)
+    s = dg.Solver(m, prob, p, prec)
+    for kind in ((1, 2, 3) if p <= 3 else (2, 3)):
+        s.set_kernel(kind)
+        err = rel_max(s.rhs(u, dt=0.01), ref)
+        print("  kernel %d rel max err %.3e" % (kind, err))
+        assert err <= (1e-13 if prec == dg.DG_FP64 else 1e-4)
+
+
+@pytest.mark.parametrize("prec", [dg.DG_FP64, dg.DG_FP32])
+@pytest.mark.parametrize("name,p", [("cylinder", 2), ("cylinder1", 1), ("cylinder1", 3), ("cylinder1", 4)])
+def test_cylinder_rk2_run_vs_oracle(name, p, prec):
+    """The reference's run of the cylinder: RK2 from the uniform state (the
+    impulsive start around the wall), to t = 0.05."""
+    m = cylinder_mesh(name)
+    prob = dg.cylinder()
+    o = orc.Oracle.from_mesh(m, p, prob)
+    u0 = o.project_ic(orc.ic_array(prob, m.period))
+    ref, n_ref, _ = o.integrate(u0, 0.05, 0.3, scheme=dg.DG_RK2)
+    s = dg.Solver(m, prob, p, prec)
+    s.project_ic()
+    n, t = s.integrate(0.05, 0.3, scheme=dg.DG_RK2)
+    err = orc.rel_l2(s.get_state(), ref)
+    print("cylinder %s p=%d prec=%d kernel %d steps %d/%d rel L2 %.3e" % (name, p, prec, s.kernel(), n, n_ref, err))
+    assert t == 0.05
+    if prec == dg.DG_FP64:
+        assert n == n_ref and err <= 1e-12
+    else:
+        assert err <= 1e-5
+    a0, a1 = o.mass(u0), np.array(s.audit()["mass"])
+    assert np.isfinite(a1).all() and s.audit()["n_bad"] == 0

@@ -79,3 +79,24 @@ def test_eval_vertices_vs_oracle(N, prec):
             got = s.eval_vertices(var)
             err = np.abs(got - want).max() / np.abs(want).max()
             assert err <= 1e-14, (p, var, err)
+
+
+@pytest.mark.parametrize("prec", [dg.DG_FP64, dg.DG_FP32])
+def test_velocity_and_pressure_views_vs_oracle(prec):
+    """dg_eval_vertices_velocity / _pressure (the reference CPU driver's
+    eval_u_velocity and eval_p): from the oracle's vertex values of the held
+    state, u = (rho u)/rho, v = (rho v)/rho, p = (gamma - 1)(E - (u^2 + v^2)/2 rho)."""
+    m = mesh(9, seed=6)
+    prob = problem_for(4)
+    for p in (1, 3, 4):
+        o = orc.Oracle.from_mesh(m, p, prob)
+        s = dg.Solver(m, prob, p, prec)
+        s.set_state(perturbed_state(o, prob, m.period, seed=p))
+        held = s.get_state()
+        rho, mu, mv, E = (o.eval_vertices(held, v) for v in range(4))
+        u, v = mu / rho, mv / rho
+        pw = 0.4 * (E - (u * u + v * v) / 2.0 * rho)
+        uv = s.eval_vertices_velocity()
+        assert np.abs(uv[..., 0] - u).max() <= 1e-13 * np.abs(u).max()
+        assert np.abs(uv[..., 1] - v).max() <= 1e-13 * np.abs(v).max()
+        assert np.abs(s.eval_vertices_pressure() - pw).max() <= 1e-13 * np.abs(pw).max()

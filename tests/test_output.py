@@ -65,3 +65,43 @@ def test_l2_error_converges_at_order_p_plus_1(p):
     print("p=%d density L2 errors %s rates %s" % (p, ["%.3e" % e for e in errs], ["%.2f" % r for r in rates]))
     for r in rates:
         assert r >= 0.85 * (p + 1), rates
+
+
+# ---------------------------------------------------------------- named views
+
+CPU_OUT = os.path.join(GOLD_DIR, "cpu_output")
+VIEWS = [("uniform_rho.out", "Density ", 1), ("uniform_E.out", "E ", 1), ("p.out", "E ", 1),
+         ("uniform_u.out", "u ", 2)]
+
+
+def view_values(path, ncomp):
+    v = dump_values(path)
+    if ncomp == 1:
+        return v  # (K, 3)
+    return v.reshape(-1, 3, 3)[:, :, :2]  # (u, v, 0) per vertex
+
+
+@pytest.mark.parametrize("fname,name,ncomp", VIEWS)
+def test_named_views_reproduce_the_reference_cpu_output(tmp_path, fname, name, ncomp):
+    """The reference's CPU Euler driver wrote these views on sv1 (the density,
+    velocity, energy and, under the view name "E ", pressure files of
+    euler/cpu/output); given their vertex values and the mesh, the writer
+    reproduces each file byte for byte."""
+    path = os.path.join(CPU_OUT, fname)
+    vals = view_values(path, ncomp)
+    m = golden_mesh("sv1")
+    assert vals.shape[0] == m.num_elem
+    out = tmp_path / fname
+    m.write_msh_view(name, vals, out)
+    assert open(out, "rb").read() == open(path, "rb").read()
+
+
+def test_reference_pressure_view_is_the_pressure_of_its_state():
+    """p.out against (gamma - 1)(E - (u^2 + v^2)/2 rho) of the same run's
+    density, velocity and energy views, vertex by vertex (6 printed digits)."""
+    rho = view_values(os.path.join(CPU_OUT, "uniform_rho.out"), 1)
+    E = view_values(os.path.join(CPU_OUT, "uniform_E.out"), 1)
+    uv = view_values(os.path.join(CPU_OUT, "uniform_u.out"), 2)
+    p = view_values(os.path.join(CPU_OUT, "p.out"), 1)
+    want = 0.4 * (E - (uv[..., 0] ** 2 + uv[..., 1] ** 2) / 2 * rho)
+    assert np.abs(p - want).max() <= 2e-5 * np.abs(p).max()

@@ -49,9 +49,11 @@ def euler_flux(U, g):
     return fx, fy, p
 
 
-def llf(a, b, n, phys, g, adv):
+def llf(a, b, n, phys, g, adv, central=False):
+    """Local Lax-Friedrichs flux; central: the same without the dissipation
+    term, (F(a) + F(b)).n / 2 (advection)."""
     if phys == 1:
-        lam = abs(adv[0] * n[0] + adv[1] * n[1])
+        lam = 0.0 if central else abs(adv[0] * n[0] + adv[1] * n[1])
         fa = np.array([adv[0] * a[0]]), np.array([adv[1] * a[0]])
         fb = np.array([adv[0] * b[0]]), np.array([adv[1] * b[0]])
     else:
@@ -66,7 +68,7 @@ def llf(a, b, n, phys, g, adv):
 
 
 def rhs(V, elem_s, sides, side_int, period, u, p, phys, coeff, vol_rule, face_rule, dt=1.0, g=1.4,
-        adv=(1.0, 1.0)):
+        adv=(1.0, 1.0), central=False):
     """vol_rule = (r, s, w) on the reference triangle (w sums to 1/2);
     face_rule = (x, w) Gauss points on [-1, 1]."""
     V = np.asarray(V).reshape(6, -1)
@@ -128,7 +130,7 @@ def rhs(V, elem_s, sides, side_int, period, u, p, phys, coeff, vol_rule, face_ru
                 x = mid + 0.5 * xi * d
                 uin = trace(e, x, np.zeros(2))
                 uout = trace(nb, x, shift)
-                Fs = llf(uin, uout, n, phys, g, adv)
+                Fs = llf(uin, uout, n, phys, g, adv, central) if central else llf(uin, uout, n, phys, g, adv)
                 rs = np.linalg.solve(A, x - v0)
                 acc -= 0.5 * ln * wi * np.outer(Fs, B.val(rs[0], rs[1]))
         out[:, :, e] = dt / J * acc
