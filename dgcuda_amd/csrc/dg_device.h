@@ -299,6 +299,10 @@ __device__ __forceinline__ void flux(const PhysP& ph, const R* U, R* fx, R* fy) 
 
 template <typename R>
 __device__ __forceinline__ R sound_speed(const PhysP& ph, const R* U) {
+    // rounded as written, like the oracle (no contraction into fused
+    // multiply-adds): under DG_QUIRK_FLOAT_SQRT the float square root must
+    // see the same double
+#pragma clang fp contract(off)
     const R rho = U[0], ru = U[1], rv = U[2], E = U[3];
     const R g = (R)ph.gamma;
     R p = (g - R(1)) * (E - (ru * ru + rv * rv) / R(2) / rho);
@@ -362,6 +366,15 @@ __device__ __forceinline__ void normal_flux_speed(const PhysP& ph, const R* U, R
     F[1] = ru * un + p * nx;
     F[2] = rv * un + p * ny;
     F[3] = (E + p) * un;
+    if constexpr (sizeof(R) == 8) {
+        // float-rounded sound speed (DG_QUIRK_FLOAT_SQRT): its argument in
+        // the reference's order (eval_c, euler.cu:78-81), so the float
+        // rounding of sqrtf sees the same double as the oracle's
+        if (ph.quirks & 1) {
+            spd = fabs(un) + sound_speed<R>(ph, U);
+            return;
+        }
+    }
     spd = fabs(un) + nf_sqrt(g * p * ir, ph.quirks);
 }
 

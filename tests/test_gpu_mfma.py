@@ -143,3 +143,25 @@ def test_tiny_meshes_every_kernel(n):
                 s.set_kernel(kind)
                 err = rel_max(s.rhs(u, dt=0.01), ref)
                 assert err <= 2e-5, (kind, err)
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_float_sqrt_quirk_fp64_many_states(p):
+    """Under DG_QUIRK_FLOAT_SQRT the sound speed goes through a float square
+    root, so an argument that differs from the oracle's in its last bits can
+    flip the float rounding (a 6e-8 relative change of the wave speed).  The
+    FP64 kernels form it in the reference's order without contraction: every
+    kernel matches the oracle to 1e-13 on 10 seeded states."""
+    m = shuffled_mesh(10, seed=21)
+    prob = problem_for(4, quirks=dg.DG_QUIRK_FLOAT_SQRT)
+    o = orc.Oracle.from_mesh(m, p, prob)
+    s = dg.Solver(m, prob, p, dg.DG_FP64)
+    worst = 0.0
+    for seed in range(10):
+        u = perturbed_state(o, prob, m.period, seed=100 + seed, amp=3e-2)
+        ref = o.rhs(u, dt=0.01)
+        for kind in (1, 2, 3):
+            s.set_kernel(kind)
+            worst = max(worst, rel_max(s.rhs(u, dt=0.01), ref))
+    print("p=%d worst rel max err %.3e" % (p, worst))
+    assert worst <= 1e-13
