@@ -4,7 +4,7 @@ do not need a full oracle run, plus the oracle on the first steps.
   C2  vortex p = 2, 128x128 squares (32,768 triangles), t = 0..2: FP64 within
       1e-12 of the oracle over the first 10 steps; the full run in FP64 and
       FP32 (FP32 within 1e-5 of FP64), mass conserved to round-off (FP64
-      1e-13, FP32 audit of an FP32 state 2e-6), and the state at t = 2 close
+      1e-13, FP32 audit of an FP32 state 1e-5), and the state at t = 2 close
       to the exact solution, the vortex carried by the background flow
       (u = v = 1) to (7, 7): relative L2 distance below 1e-5 from its
       projection (the oracle measures 1.3e-4 at 32x32 and 1.9e-5 at 64x64,
@@ -50,7 +50,7 @@ def test_c2_full_size():
     assert t64 == t_end and t32 == t_end and err32 <= 1e-5
     m0 = o.mass(u0)
     assert np.all(np.abs(np.array(s64.audit()["mass"]) - m0) <= 1e-13 * np.abs(m0))
-    assert np.all(np.abs(np.array(s32.audit()["mass"]) - m0) <= 2e-6 * np.abs(m0))
+    assert np.all(np.abs(np.array(s32.audit()["mass"]) - m0) <= 1e-5 * np.abs(m0))
     moved = dg.euler_vortex(((5.0, 7.0, 7.0),))
     exact = dg.Solver(m, moved, p, dg.DG_FP64)
     exact.project_ic()
