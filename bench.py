@@ -48,24 +48,6 @@ def algorithmic_bytes_per_elem(N, n_p, s):
     return 4.25 * N * n_p * s + 110.0
 
 
-def kernel_bytes_per_elem(N, n_p, s, kernel):
-    """Bytes the kernels of this build must move per element and stage
-    (each input read once, each output written once; halo re-reads not
-    counted).  Matrix-core path (kernel 2/3), RK4 in stage states: state
-    passes 3.0*N*n_p*s (stage 1 reads u_n and writes u2; stage 2 reads u2
-    and u_n and writes u3 and W = (2 u_n + (u2 - u_n) + dt k2)/3; stage 3
-    reads u3 and u_n and writes u4; stage 4 reads u4 and W and writes
-    u_n+1: 2, 4, 3, 3 arrays), plus the packed geometry: per face one
-    record of 4 values (nx, ny, len, neighbour word), per element a map of
-    4 values (xr, yr, xs, ys) and J, 17*s bytes (68 in FP32).  Lane-per-element
-    path (kernel 1): the reference's grouping, where stage 4 also reads u_n
-    (4.25 passes), geometry 14*s (xr, yr, xs, ys, J; per face nx, ny, len)
-    and 3 32-bit neighbour words."""
-    if kernel >= 2:
-        return 3.0 * N * n_p * s + (3 * 4 + 4 + 1) * s
-    return 4.25 * N * n_p * s + 14 * s + 3 * 4
-
-
 def start_cpu_baseline(c3_steps):
     """The FP64 serial oracle on one pinned host core (oracle/cpu_baseline.py)
     in a child process, started before this process touches the GPU and run
@@ -125,10 +107,23 @@ def load_traffic(args, kernel):
     return d.get("bytes_per_launch")
 
 
-def batch_summary(steps=20, steps64=4):
+def bounds(detail, elapsed, copy_gbs):
+    """The bounds of batching a workload that took `elapsed` s one solve at a
+    time (workloads._ceiling): at the single-solve rates, and at the copy
+    kernel's HBM rate over the bytes the kernels must move."""
+    floor = detail["hbm_bytes"] / (copy_gbs * 1e9)
+    return dict(best_single_rate_by_p={k: float("%.4g" % v) for k, v in detail["best_rate_by_p"].items()},
+                ideal_at_single_rates_s=float("%.4g" % detail["ideal_s"]),
+                ratio_at_single_rates=round(elapsed / detail["ideal_s"], 3),
+                hbm_bytes=float("%.4g" % detail["hbm_bytes"]),
+                hbm_floor_s=float("%.4g" % floor), ratio_at_hbm_floor=round(elapsed / floor, 3))
+
+
+def batch_summary(copy_gbs, steps=20, steps64=4):
     """Ragged batch: 64 solves (500-200k triangles, p 1..4, half advection
     and half Euler vortices) in one persistent launch per RK stage, against
-    the same solves one after another on the single-solve path."""
+    the same solves one after another on the single-solve path, and the
+    bounds of any batching of them."""
     import dgcuda_amd as dg
     from dgcuda_amd import workloads as W
 
@@ -138,19 +133,22 @@ def batch_summary(steps=20, steps64=4):
     for prec, name, n in ((dg.DG_FP32, "fp32", steps), (dg.DG_FP64, "fp64", steps64)):
         el, work, b = W.run_batch(cases, n, prec)
         b.close()
-        el2, _, ss = W.run_sequential(cases, n, prec)
+        detail = {}
+        el2, _, ss = W.run_sequential(cases, n, prec, detail=detail)
         for x in ss:
             x.close()
         out[name] = dict(steps=n, batched_dof_stage_per_s=float("%.4g" % (work / el)),
                          one_at_a_time_dof_stage_per_s=float("%.4g" % (work / el2)),
-                         speedup=round(el2 / el, 3))
+                         speedup=round(el2 / el, 3), bounds=bounds(detail, el2, copy_gbs))
+    out["kernel_choice"] = ("fixed per order as the autotuner picks at large sizes: lane items (one element per "
+                            "lane) for p <= 2, matrix-core tiles for p >= 3")
     return out
 
 
-def serving_summary():
+def serving_summary(copy_gbs):
     """Served (continuous batching) against plain one-at-a-time solves on the
     same seeded Poisson request streams: the 1k-500k-triangle mix and the
-    small-solve mix."""
+    small-solve mix, and the bounds of any batching of them."""
     from dgcuda_amd import workloads as W
 
     out = {}
@@ -161,7 +159,10 @@ def serving_summary():
         def r(d):
             return {k: (float("%.4g" % v) if isinstance(v, float) else r(v) if isinstance(v, dict) else v)
                     for k, v in d.items()}
-        out[name] = dict(served=r(served), plain=r(plain),
+        b = bounds(plain, plain["makespan_s"], copy_gbs)
+        for k in ("best_rate_by_p", "ideal_s", "ideal_ratio", "hbm_bytes"):
+            plain.pop(k)
+        out[name] = dict(served=r(served), plain=r(plain), bounds=b,
                          solves_per_s_ratio=round(served["solves_per_s"] / plain["solves_per_s"], 3))
     return out
 
@@ -251,7 +252,7 @@ def main():
     bpe = algorithmic_bytes_per_elem(N, n_p, s_bytes)
     bytes_launch = bpe * K
     achieved = bytes_launch / (stage_ms * 1e-3) / 1e9
-    kbytes_launch = kernel_bytes_per_elem(N, n_p, s_bytes, s.kernel()) * K
+    kbytes_launch = W.kernel_bytes_per_elem(N, n_p, s_bytes, s.kernel()) * K
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                 frac=round(achieved / PEAK_HBM_GBS, 4), traffic=load_traffic(args, s.kernel()),
                 kernel="stage (fused volume + LLF faces + M^-1 + RK update)",
@@ -296,9 +297,9 @@ def main():
             reference="GPU FP64 run to the same time; the FP64 GPU path matches the FP64 oracle to <=1e-12 in "
                       "tests. rel_l2_fp32_geometry_only: FP64 state and kernels with the geometry rounded "
                       "through FP32, as the FP32 path stores it")
-    batch = None if args.no_batch else batch_summary()
-    serving = None if args.no_serving else serving_summary()
     bw_ms, bw, bw_kernel = dg.bandwidth_probe(1 << 30, 10, with_variant=True)
+    batch = None if args.no_batch else batch_summary(bw)
+    serving = None if args.no_serving else serving_summary(bw)
     cpu = finish_cpu_baseline(cpu_proc) if cpu_proc is not None else None
     traffic = roof["traffic"]
     stale = traffic == "stale"

@@ -88,6 +88,48 @@ def dof_stage_per_step(mesh, prob, p):
     return mesh.num_elem * n_modes(p) * prob.physics * 4
 
 
+def kernel_bytes_per_elem(N, n_p, s, kernel):
+    """Bytes the kernels of this build must move per element and stage
+    (each input read once, each output written once; halo re-reads not
+    counted).  Matrix-core path (kernel 2/3), RK4 in stage states: state
+    passes 3.0*N*n_p*s (stage 1 reads u_n and writes u2; stage 2 reads u2
+    and u_n and writes u3 and W = (2 u_n + (u2 - u_n) + dt k2)/3; stage 3
+    reads u3 and u_n and writes u4; stage 4 reads u4 and W and writes
+    u_n+1: 2, 4, 3, 3 arrays), plus the packed geometry: per face one
+    record of 4 values (nx, ny, len, neighbour word), per element a map of
+    4 values (xr, yr, xs, ys) and J, 17*s bytes (68 in FP32).  Lane-per-element
+    path (kernel 1): the reference's grouping, where stage 4 also reads u_n
+    (4.25 passes), geometry 14*s (xr, yr, xs, ys, J; per face nx, ny, len)
+    and 3 32-bit neighbour words."""
+    if kernel >= 2:
+        return 3.0 * N * n_p * s + (3 * 4 + 4 + 1) * s
+    return 4.25 * N * n_p * s + 14 * s + 3 * 4
+
+
+def bytes_per_step(mesh, prob, p, precision, kernel):
+    """HBM bytes of one RK4 step (4 stages) of one solve on kernel `kernel`."""
+    s = 4 if precision == DG_FP32 else 8
+    return 4 * mesh.num_elem * kernel_bytes_per_elem(prob.physics, n_modes(p), s, kernel)
+
+
+def _ceiling(runs, elapsed, detail):
+    """Bounds of any batching of `runs` = [(p, DOF*stage, HBM bytes, seconds
+    alone)], which took `elapsed` one at a time:
+      ideal_s: the work at the best rate a single solve of each order p
+        reached (that of the largest); small solves alone are bound by launch
+        latency, which a batch amortises, so a batch can beat this figure;
+      hbm_bytes: the bytes the kernels must move; divided by a measured copy
+        rate this is the least time any schedule of these kernels takes."""
+    best = {}
+    for p, work, _, ti in runs:
+        best[p] = max(best.get(p, 0.0), work / ti)
+    ideal = sum(work / best[p] for p, work, _, _ in runs)
+    detail.update(best_rate_by_p={str(p): best[p] for p in sorted(best)}, ideal_s=ideal,
+                  ideal_ratio=elapsed / ideal, hbm_bytes=float(sum(b for _, _, b, _ in runs)),
+                  work_dof_stage=float(sum(w for _, w, _, _ in runs)),
+                  alone_s=float(sum(t for _, _, _, t in runs)))
+
+
 def run_batch(cases, steps, precision=DG_FP32, use_graph=True, warmup=2):
     """All cases in one batch, `steps` RK4 steps each; returns (seconds, DOF*stage)."""
     b = Batch(precision)
@@ -102,20 +144,29 @@ def run_batch(cases, steps, precision=DG_FP32, use_graph=True, warmup=2):
     return el, work, b
 
 
-def run_sequential(cases, steps, precision=DG_FP32, use_graph=True, warmup=2):
-    """The same cases one after another on the single-solve path."""
+def run_sequential(cases, steps, precision=DG_FP32, use_graph=True, warmup=2, detail=None):
+    """The same cases one after another on the single-solve path.  With a
+    dict `detail`, also each solve's time alone and the bounds of any
+    batching of these cases (_ceiling)."""
     solvers = []
     for m, prob, p in cases:
         s = Solver(m, prob, p, precision)
         s.project_ic()
         s.run_steps(warmup, 0.3, use_graph=use_graph)
         solvers.append(s)
+    times = []
     t0 = time.perf_counter()
     for s in solvers:
-        s.run_steps(steps, 0.3, use_graph=use_graph)
+        ti = time.perf_counter()
+        s.run_steps(steps, 0.3, use_graph=use_graph)  # returns once the steps are done
+        times.append(time.perf_counter() - ti)
     el = time.perf_counter() - t0
-    work = sum(dof_stage_per_step(m, prob, p) for m, prob, p in cases) * steps
-    return el, work, solvers
+    works = [dof_stage_per_step(m, prob, p) * steps for m, prob, p in cases]
+    if detail is not None:
+        runs = [(p, w, bytes_per_step(m, prob, p, precision, s.kernel()) * steps, ti)
+                for (m, prob, p), w, s, ti in zip(cases, works, solvers, times)]
+        _ceiling(runs, el, detail)
+    return el, sum(works), solvers
 
 
 def _latency_stats(lat):
@@ -149,13 +200,8 @@ def run_served(reqs, precision=DG_FP32, max_slots=64, hbm_budget=64 << 30, steps
 def run_plain(reqs, precision=DG_FP32, keep=()):
     """One request at a time on the single-solve path, in arrival order:
     build the mesh and solver, project, integrate to t_end, audit, free.
-
-    Also the ceiling of any batching of this stream: each request's work
-    (DOF*stage = K n_p N 4 steps) and the time its integration took alone
-    give per order p the best rate a single solve reached (that of the
-    largest solves, which fill the GPU); the stream's work at those rates is
-    the least time any server can take with these kernels, and the plain
-    makespan over it the largest served/plain ratio possible."""
+    Also each request's integration time alone and the bounds of any
+    batching of this stream (_ceiling)."""
     t0 = time.perf_counter() + 0.05
     lat, states, runs = [], {}, []
     for k, r in enumerate(reqs):
@@ -172,17 +218,12 @@ def run_plain(reqs, precision=DG_FP32, keep=()):
         s.audit()
         if k in keep:
             states[k] = s.get_state()
-        runs.append((r.p, m.num_elem * n_modes(r.p) * r.prob.physics * 4 * n, ti))
+        runs.append((r.p, dof_stage_per_step(m, r.prob, r.p) * n,
+                     bytes_per_step(m, r.prob, r.p, precision, s.kernel()) * n, ti))
         s.close()
         lat.append(time.perf_counter() - (t0 + r.arrival_s))
     makespan = time.perf_counter() - t0
     out = dict(solves=len(reqs), makespan_s=makespan, solves_per_s=len(reqs) / makespan)
     out.update({"latency_" + k: v for k, v in _latency_stats(lat).items()})
-    best = {}
-    for p, work, ti in runs:
-        best[p] = max(best.get(p, 0.0), work / ti)
-    ideal = sum(work / best[p] for p, work, _ in runs)
-    out.update(work_dof_stage=float(sum(w for _, w, _ in runs)), integrate_s=float(sum(t for _, _, t in runs)),
-               best_rate_by_p={str(p): best[p] for p in sorted(best)}, ideal_makespan_s=ideal,
-               ceiling_ratio=makespan / ideal)
+    _ceiling(runs, makespan, out)
     return out, states
