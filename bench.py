@@ -201,6 +201,7 @@ def main():
         cpu_proc = start_cpu_baseline(args.cpu_c3_steps)
 
     import dgcuda_amd as dg
+    from dgcuda_amd import workloads as W
 
     prec = dg.DG_FP32 if args.precision == "fp32" else dg.DG_FP64
     s_bytes = 4 if prec == dg.DG_FP32 else 8
