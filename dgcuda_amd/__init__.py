@@ -62,7 +62,7 @@ class AuditOut(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("max_slots", C.c_int), ("hbm_budget", C.c_int64),
                 ("steps_per_round", C.c_int), ("cfl", C.c_double), ("use_graph", C.c_int),
-                ("solo_min_elems", C.c_int), ("prep_threads", C.c_int)]
+                ("solo_min_elems", C.c_int), ("prep_threads", C.c_int), ("fifo_min_elems", C.c_int)]
 
 
 class Request(C.Structure):
@@ -533,15 +533,20 @@ class Batch:
 # A solve of this many triangles fills the GPU by itself (about one tile per
 # resident workgroup): larger solves run their own kernels inside the step.
 SOLO_MIN_ELEMS = 65536
+# Solves of this many triangles and more advance one at a time in arrival
+# order (the smaller ones beside them): each keeps the GPU busy alone, and
+# advancing them together would make the first to arrive wait for the later
+# ones (served p99 latency above the plain one-at-a-time path's).
+FIFO_MIN_ELEMS = 262144
 
 
 class Server:
     """Continuous-batching solve server."""
 
     def __init__(self, precision=DG_FP32, max_slots=64, hbm_budget=64 << 30, steps_per_round=8, cfl=0.3,
-                 use_graph=True, solo_min_elems=SOLO_MIN_ELEMS, prep_threads=4):
+                 use_graph=True, solo_min_elems=SOLO_MIN_ELEMS, prep_threads=4, fifo_min_elems=FIFO_MIN_ELEMS):
         cfg = ServerConfig(precision, max_slots, hbm_budget, steps_per_round, cfl, 1 if use_graph else 0,
-                           solo_min_elems, prep_threads)
+                           solo_min_elems, prep_threads, fifo_min_elems)
         self._h = C.c_void_p()
         _check(lib().dg_server_create(C.byref(cfg), C.byref(self._h)))
 

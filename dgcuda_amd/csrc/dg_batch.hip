@@ -29,7 +29,7 @@ struct SlotDesc {
     R *c, *acc, *ua, *ub;
     const R *geo_e, *geo_f;
     const int* nbr;
-    const uint16_t* fword;
+    const uint32_t* fword;
     const int* halo;
     const void* face4;
     const void* metric4;
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float
     extern __shared__ float4 lds4[];
     __shared__ double sh[8];
     floatx4* fd = reinterpret_cast<floatx4*>(lds4);
-    float* ts = reinterpret_cast<float*>(fd + MFMA_TILE * 3);
+    float* ts = tile_traces<floatx4, float>(fd);
     __shared__ floatx4 tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
     const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
     batch_lane_items<float, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256, 1) k_batch_stage_f64(const SlotDesc<doubl
     extern __shared__ float4 lds4[];
     __shared__ double sh[8];
     doublex4* fd = reinterpret_cast<doublex4*>(lds4);
-    double* ts = reinterpret_cast<double*>(fd + MFMA_TILE * 3);
+    double* ts = tile_traces<doublex4, double>(fd);
     __shared__ doublex4 tabA[MFMA_A_SLOTS * 64];
     const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
     batch_lane_items<double, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);

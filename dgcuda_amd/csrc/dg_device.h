@@ -98,10 +98,11 @@ __device__ __forceinline__ void lam_commit(double mx, unsigned long long* bits) 
 
 // Start of a step on the device: dt = cfl * min_r / max lambda / (2p+1),
 // clamped to t_end, t advanced before the stages (time_integrator.cu:254-270);
-// reads and clears the accumulated maximum wave speed.  One thread.
+// reads the accumulated maximum wave speed and clears it for the step's last
+// stage to fill.  A step not taken (t_end reached) leaves it: it still holds
+// the wave speed of the current state, which a later run starts from.  One
+// thread.
 __device__ __forceinline__ void dt_update(Ctrl* ctrl) {
-    const double mx = __longlong_as_double((long long)ctrl->lam_bits);
-    ctrl->lam_bits = 0ull;
     if (ctrl->done) {
         ctrl->active = 0;
         return;
@@ -113,6 +114,8 @@ __device__ __forceinline__ void dt_update(Ctrl* ctrl) {
         ctrl->dt = 0.0;
         return;
     }
+    const double mx = __longlong_as_double((long long)ctrl->lam_bits);
+    ctrl->lam_bits = 0ull;
     double dt = ctrl->cfl * ctrl->min_r / mx / ctrl->two_p1;
     if (ctrl->has_end && t + dt > ctrl->t_end) {
         dt = ctrl->t_end - t;

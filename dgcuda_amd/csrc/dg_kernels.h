@@ -93,14 +93,21 @@ template <typename R>
 hipError_t launch_stage_valu(int N, int p, const StageArgs<R>& a, dim3 grid, hipStream_t st);
 
 // Extra per-solve data of the matrix-core kernel: per (element, face) a
-// 16-bit word {bits 0..11 LDS slot of the neighbour (own tile element or
-// halo entry), bits 12..13 neighbour's local face, bit 14 left element},
+// 32-bit word {bits 0..11 LDS slot of the neighbour (own tile element or
+// halo entry), bits 12..13 neighbour's local face, bit 14 left element,
+// bit 15 boundary side, bits 16..23 the side's index in its tile's side
+// list, bit 24 set if the element computes the side's flux (FW_OWNER)},
 // stored [3][Kp]; per tile the halo element list [ntiles][hcap]; and the
 // packed geometry in the bulk precision: per (element, face) 4 values
 // (nx, ny, len, word) [Kp][3], per element 4 values (xr, yr, xs, ys) [Kp],
-// padding elements replicating element K-1.
+// padding elements replicating element K-1 (without FW_OWNER).
+// A tile computes each of its sides once: the boundary sides, the sides to
+// its halo, and of the sides between two of its elements the one whose
+// record has the left element.
+constexpr int FW_TASK_SHIFT = 16;
+constexpr uint32_t FW_OWNER = 1u << 24;
 struct MfmaArgs {
-    const uint16_t* fword;
+    const uint32_t* fword;
     const int* halo;
     const void* face4;
     const void* metric4;
@@ -108,7 +115,7 @@ struct MfmaArgs {
     int pad;
 };
 template <typename R>
-hipError_t launch_mfma_pack(const R* geo_e, const R* geo_f, const uint16_t* fword, int K, int Kp, void* face4,
+hipError_t launch_mfma_pack(const R* geo_e, const R* geo_f, const uint32_t* fword, int K, int Kp, void* face4,
                             void* metric4, hipStream_t st);
 
 // matrix-core element kernel (FP32 or FP64, p = 1..4, state in the tile

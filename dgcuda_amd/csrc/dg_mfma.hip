@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256, mfma_minw<R>()) k_stage_mfma(StageArgs<R>
     constexpr bool LAST = (KIND == ST_RK4_4 || KIND == ST_RK2_2);
     extern __shared__ float4 lds4[];
     V* fd = reinterpret_cast<V*>(lds4);
-    R* ts = reinterpret_cast<R*>(fd + MFMA_TILE * 3);
+    R* ts = tile_traces<V, R>(fd);
     __shared__ double shmax[8];
 
     if (!a.ctrl->active) return;
@@ -136,7 +136,7 @@ size_t mfma_lds_bytes(int N, int hcap, int rs) { return mfma_tile_lds_bytes(N, h
 // value).
 template <typename R>
 __global__ void __launch_bounds__(256) k_mfma_pack(const R* __restrict__ geo_e, const R* __restrict__ geo_f,
-                                                  const uint16_t* __restrict__ fword, int K, int Kp,
+                                                  const uint32_t* __restrict__ fword, int K, int Kp,
                                                   typename MfmaT<R>::V* __restrict__ face4,
                                                   typename MfmaT<R>::V* __restrict__ metric4) {
     using V = typename MfmaT<R>::V;
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(256) k_mfma_pack(const R* __restrict__ geo_e, 
     if (e >= Kp) return;
     const size_t s = (size_t)min(e, K - 1), kp = (size_t)Kp;
     for (int f = 0; f < 3; f++) {
-        const int wd = (int)fword[f * kp + s];
+        const int wd = (int)(fword[f * kp + s] & (e < K ? ~0u : ~FW_OWNER));
         R wv;
         if constexpr (sizeof(R) == 4)
             wv = __int_as_float(wd);
@@ -156,16 +156,16 @@ __global__ void __launch_bounds__(256) k_mfma_pack(const R* __restrict__ geo_e, 
 }
 
 template <typename R>
-hipError_t launch_mfma_pack(const R* geo_e, const R* geo_f, const uint16_t* fword, int K, int Kp, void* face4,
+hipError_t launch_mfma_pack(const R* geo_e, const R* geo_f, const uint32_t* fword, int K, int Kp, void* face4,
                             void* metric4, hipStream_t st) {
     using V = typename MfmaT<R>::V;
     hipLaunchKernelGGL((k_mfma_pack<R>), dim3((Kp + 255) / 256), dim3(256), 0, st, geo_e, geo_f, fword, K, Kp,
                        (V*)face4, (V*)metric4);
     return hipGetLastError();
 }
-template hipError_t launch_mfma_pack<float>(const float*, const float*, const uint16_t*, int, int, void*, void*,
+template hipError_t launch_mfma_pack<float>(const float*, const float*, const uint32_t*, int, int, void*, void*,
                                             hipStream_t);
-template hipError_t launch_mfma_pack<double>(const double*, const double*, const uint16_t*, int, int, void*, void*,
+template hipError_t launch_mfma_pack<double>(const double*, const double*, const uint32_t*, int, int, void*, void*,
                                              hipStream_t);
 
 // workgroups of the matrix-core kernel that fit on the device at once

@@ -11,16 +11,20 @@
 //   volume projection     R[m][e]  = sum_q Gx[m][q] Fr[q][e] + Gy[m][q] Fs[q][e]  (K = 32)
 //   surface projection    R[m][e] += sum_pt S[m][pt] w    G[pt][e]   (K = 16)
 // where pt runs over the 3*(p+1) face points (padded to 16).  Each lane
-// computes the fluxes of the points its accumulator registers hold; the
-// reduction dimension of the projections is ordered the same way, so no
+// computes the volume fluxes of the points its accumulator registers hold;
+// the reduction dimension of the projections is ordered the same way, so no
 // data moves between lanes between the products.
 //
 // Neighbour traces come through LDS: every element's own face traces are
 // written to LDS, and the tile's halo (neighbours outside the tile, listed
-// per tile on the host) is traced into extra LDS slots.  Each face flux is
-// evaluated by both of its elements in the side's canonical orientation with
-// identical operands, so the two contributions are bit-identical with
-// opposite signs: race-free, no atomics, no face buffers in HBM.
+// per tile on the host) is traced into extra LDS slots.  The tile then
+// evaluates the LLF flux of each of its sides once, spread evenly over its
+// 256 lanes: a side between two of its elements is computed by one of them
+// and written to both (the negation is exactly what the other would
+// compute); a side to the halo is computed by the tile's element, and again
+// by the neighbour's own tile, with identical operands, so the two
+// contributions are bit-identical with opposite signs: race-free, no
+// atomics, no face buffers in HBM.
 #pragma once
 
 #include "dg_kernels.h"
@@ -61,11 +65,19 @@ struct MfmaT<double> {
 constexpr int MFMA_TILE = 64;
 
 // LDS of a tile: face records [64][3] of 4 values (nx, ny, len, packed
-// word) then traces [(64 + hcap)][17*N] values (16 points + 1 padding row so
-// consecutive slots start in different banks), N variables innermost;
-// `rs` = bytes per value (4 FP32, 8 FP64).
+// word), the tile's side list (MFMA_TASK_BYTES: 192 entries of 8 bytes and
+// 4 per-wavefront counts), then traces [(64 + hcap)][17*N] values (16
+// points + 1 padding row so consecutive slots start in different banks), N
+// variables innermost; `rs` = bytes per value (4 FP32, 8 FP64).
+constexpr int MFMA_TASK_BYTES = 192 * 8 + 32;
 __host__ __device__ constexpr size_t mfma_tile_lds_bytes(int N, int hcap, int rs = 4) {
-    return (size_t)MFMA_TILE * 3 * 4 * rs + (size_t)(MFMA_TILE + hcap) * 17 * N * rs;
+    return (size_t)MFMA_TILE * 3 * 4 * rs + MFMA_TASK_BYTES + (size_t)(MFMA_TILE + hcap) * 17 * N * rs;
+}
+
+// the traces of a tile's LDS, after the face records `fd` and the side list
+template <typename V, typename R>
+__device__ __forceinline__ R* tile_traces(V* fd) {
+    return reinterpret_cast<R*>(reinterpret_cast<unsigned char*>(fd + MFMA_TILE * 3) + MFMA_TASK_BYTES);
 }
 
 // State arrays are in the tile layout (StateIx, tiled): lane j + 16g of a
@@ -301,6 +313,67 @@ __device__ __forceinline__ int face_word(double x) { return (int)x; }
 __device__ __forceinline__ float inv_jacobian(float J) { return __builtin_amdgcn_rcpf(J); }
 __device__ __forceinline__ double inv_jacobian(double J) { return 1.0 / J; }
 
+// LLF flux of one point of one side: face record `r`, the element's trace
+// at the point `own` and the neighbour's `nbt`, in the orientation of the
+// element; written over `own`, and its negation over `nbt` when the
+// neighbour is an element of the tile.  For a boundary side, `nw` is the
+// side's neighbour word (the ghost data's index) and jo the point.
+template <int N, int NQ1, typename R, bool BND>
+__device__ __forceinline__ void side_flux(const StageArgs<R>& a, const typename MfmaT<R>::V r, R* own, R* nbt,
+                                          int nw, int jo) {
+    using V = typename MfmaT<R>::V;
+    const int wd = face_word(r[3]);
+    const bool left = (wd >> 14) & 1;
+    const bool inner = (wd & 0xFFF) < MFMA_TILE;
+    // boundary side (the element is its left element, the slot its own): the
+    // ghost state of the point replaces the neighbour trace, which is not
+    // read.  Only the BND instantiation carries this code, so the kernels
+    // of meshes without boundary sides keep their register allocation.
+    const bool bside = BND && ((wd >> 15) & 1);
+    if constexpr (BND)
+        if (bside) nbt = own;
+    R G[N];
+    if constexpr (N == 4) {
+        // F(UL).n + F(UR).n, max of the two speeds and UL - UR with the sign
+        // of this side folded into the speed
+        R uo[N], un[N], Fo[N], Fb[N], so, sb;
+        const V qo = *reinterpret_cast<const V*>(own), qn = *reinterpret_cast<const V*>(nbt);
+#pragma unroll
+        for (int v = 0; v < N; v++) {
+            uo[v] = qo[v];
+            un[v] = qn[v];
+        }
+        if constexpr (BND)
+            if (bside) boundary_ghost<N, NQ1, R>(a.bnd, nw, jo, uo, un);
+        normal_flux_speed<R>(a.phys, uo, r[0], r[1], Fo, so);
+        normal_flux_speed<R>(a.phys, un, r[0], r[1], Fb, sb);
+        const R lm = fmax(so, sb);
+        const R lms = left ? lm : -lm;
+        const R cf = left ? R(-0.25) * r[2] : R(0.25) * r[2];
+#pragma unroll
+        for (int v = 0; v < N; v++) G[v] = cf * fma(lms, uo[v] - un[v], Fo[v] + Fb[v]);
+        *reinterpret_cast<V*>(own) = V{G[0], G[1], G[2], G[3]};
+        if (inner && !bside) *reinterpret_cast<V*>(nbt) = V{-G[0], -G[1], -G[2], -G[3]};
+    } else {
+        R UL[N], UR[N];
+        {
+            const R uo = own[0], un = nbt[0];
+            UL[0] = left ? uo : un;
+            UR[0] = left ? un : uo;
+        }
+        if constexpr (BND)
+            if (bside) boundary_ghost<N, NQ1, R>(a.bnd, nw, jo, UL, UR);
+        R fxl[N], fyl[N], fxr[N], fyr[N];
+        flux<N, R>(a.phys, UL, fxl, fyl);
+        flux<N, R>(a.phys, UR, fxr, fyr);
+        const R lm = llf_speed<N, R>(a.phys, UL, UR, r[0], r[1]);
+        const R coef = left ? R(-0.5) * r[2] : R(0.5) * r[2];
+        G[0] = coef * (R(0.5) * ((fxl[0] + fxr[0]) * r[0] + (fyl[0] + fyr[0]) * r[1] + lm * (UL[0] - UR[0])));
+        own[0] = G[0];
+        if (inner && !bside) nbt[0] = -G[0];
+    }
+}
+
 // Software-pipelined body of the persistent single-solve kernel.  On entry
 // `st` and `gc` hold the state and geometry of `tile`, and `hid` the halo
 // ids of `next`.  Once the volume interpolation has consumed the state, the
@@ -373,10 +446,29 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
             for (int i = 0; i < 4; i++) ts[slot * TS + M::pos(g, i) * N + v] = acc[i];
         }
     }
-    if (threadIdx.x < TILE * 3) fd[threadIdx.x] = gc.f;
-    __syncthreads();
+    // face records, and the side list: the thread of record t = 3 el + f of
+    // a side this tile computes writes entry (word bits 16..23) of the list,
+    // {offset of the element's trace at point 0 of the face | offset of the
+    // neighbour's trace at its last point << 16, t}; per wavefront the count
+    uint2* task = reinterpret_cast<uint2*>(fd + TILE * 3);
+    int* ntask4 = reinterpret_cast<int*>(task + TILE * 3);
+    {
+        const int t = threadIdx.x;
+        const int wd = face_word(gc.f[3]);
+        const bool owner = t < TILE * 3 && (wd & FW_OWNER);
+        if (t < TILE * 3) fd[t] = gc.f;
+        if (owner) {
+            const int eo = t / 3, f = t - 3 * eo;
+            const int slot = wd & 0xFFF, fn = (wd >> 12) & 3;
+            const unsigned oo = (unsigned)(eo * TS + f * NQ1 * N);
+            const unsigned no = (unsigned)(slot * TS + (fn * NQ1 + NQ1 - 1) * N);
+            task[(wd >> FW_TASK_SHIFT) & 0xFF] = make_uint2(oo | (no << 16), (unsigned)t);
+        }
+        const unsigned long long bm = __ballot(owner);
+        if (lane == 0) ntask4[w] = __popcll(bm);
+    }
 
-    // ---- phase 2: volume, faces, inverse mass, RK update ----
+    // ---- phase 2: volume, side fluxes, surface, inverse mass, RK update ----
     V U[N];
     const V Avol = tabA[MFMA_A_VOL * 64 + lane];
 #pragma unroll
@@ -446,68 +538,55 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
             }
         }
     }
+    __syncthreads();
+
+    // ---- phase 2b: the LLF flux of each side of the tile, once ----
+    // (after the volume terms, which use only this wave's registers, so the
+    // waves meet at the barrier with the next tile's loads long in flight)
+    // Work item (side k of the list, point jo); the 64-item blocks of a
+    // round go to the wavefronts in an order that rotates with the tile, so
+    // the partial last round falls to each wavefront in turn.  The flux is
+    // computed in the orientation of the record's element and written over
+    // that element's trace at the point; a side between two elements of the
+    // tile also writes its negation over the other element's trace, which is
+    // bit for bit what that element would compute (the sums and maxima of the
+    // two sides' terms commute and the sign flips are exact).  Each trace
+    // entry of the tile's elements belongs to one side and one point, so the
+    // item that reads it is the only one that writes it.
+    {
+        const int nitem = (ntask4[0] + ntask4[1] + ntask4[2]) * NQ1;
+        const int wblk = ((w + tile) & 3) * 64;
+        for (int r0 = 0; r0 < nitem; r0 += 256) {
+            if (r0 + wblk >= nitem) continue;  // uniform over the wavefront
+            const int it = r0 + wblk + lane;
+            if (it < nitem) {
+                const int k = it / NQ1, jo = it - k * NQ1;
+                const uint2 en = task[k];
+                int nw = 0;
+                if constexpr (BND) {
+                    const int eo = (int)en.y / 3, f = (int)en.y - 3 * eo;
+                    nw = gp(a.nbr)[f * a.Kp + base + eo];
+                }
+                side_flux<N, NQ1, R, BND>(a, fd[en.y], ts + (en.x & 0xFFFF) + jo * N, ts + (en.x >> 16) - jo * N,
+                                          nw, jo);
+            }
+        }
+    }
+    __syncthreads();
+
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        // own face point pt = pos(g, i): LLF flux in the side's orientation
+        // own face point pt = pos(g, i): its flux from phase 2b (point slots
+        // past 3(p+1) hold stale traces and meet zero)
         const int pt = M::pos(g, i);
+        const R* gq = ts + el * TS + pt * N;
         R G[N];
-        if (pt < NPT) {
-            const int f = pt / NQ1, jo = pt - f * NQ1;
-            const V r = fd[el * 3 + f];
-            const int wd = face_word(r[3]);
-            const int slot = wd & 0xFFF, fn = (wd >> 12) & 3;
-            const bool left = (wd >> 14) & 1;
-            // boundary side (the element is its left element, the slot its
-            // own): the ghost state of the point replaces the neighbour trace.
-            // Only the BND instantiation carries this code, so the kernels of
-            // meshes without boundary sides keep their register allocation.
-            const bool bside = BND && ((wd >> 15) & 1);
-            const R* own = ts + el * TS + pt * N;
-            const R* nbt = ts + slot * TS + (fn * NQ1 + NQ1 - 1 - jo) * N;
-            if constexpr (N == 4) {
-                // F(UL).n + F(UR).n, max of the two speeds and UL - UR with the
-                // sign of this side folded into the speed: sums and maxima of
-                // the two sides' terms commute and the sign flips are exact, so
-                // both elements of the side still get bit-identical values with
-                // opposite signs
-                R uo[N], un[N], Fo[N], Fb[N], so, sb;
+        if constexpr (N == 4) {
+            const V q = *reinterpret_cast<const V*>(gq);
 #pragma unroll
-                for (int v = 0; v < N; v++) {
-                    uo[v] = own[v];
-                    un[v] = nbt[v];
-                }
-                if constexpr (BND)
-                    if (bside) boundary_ghost<N, NQ1, R>(a.bnd, gp(a.nbr)[f * a.Kp + ee], jo, uo, un);
-                normal_flux_speed<R>(a.phys, uo, r[0], r[1], Fo, so);
-                normal_flux_speed<R>(a.phys, un, r[0], r[1], Fb, sb);
-                const R lm = fmax(so, sb);
-                const R lms = left ? lm : -lm;
-                const R cf = left ? R(-0.25) * r[2] : R(0.25) * r[2];
-#pragma unroll
-                for (int v = 0; v < N; v++) G[v] = cf * fma(lms, uo[v] - un[v], Fo[v] + Fb[v]);
-            } else {
-                R UL[N], UR[N];
-#pragma unroll
-                for (int v = 0; v < N; v++) {
-                    const R uo = own[v], un = nbt[v];
-                    UL[v] = left ? uo : un;
-                    UR[v] = left ? un : uo;
-                }
-                if constexpr (BND)
-                    if (bside) boundary_ghost<N, NQ1, R>(a.bnd, gp(a.nbr)[f * a.Kp + ee], jo, UL, UR);
-                R fxl[N], fyl[N], fxr[N], fyr[N];
-                flux<N, R>(a.phys, UL, fxl, fyl);
-                flux<N, R>(a.phys, UR, fxr, fyr);
-                const R lm = llf_speed<N, R>(a.phys, UL, UR, r[0], r[1]);
-                const R coef = left ? R(-0.5) * r[2] : R(0.5) * r[2];
-#pragma unroll
-                for (int v = 0; v < N; v++)
-                    G[v] = coef *
-                           (R(0.5) * ((fxl[v] + fxr[v]) * r[0] + (fyl[v] + fyr[v]) * r[1] + lm * (UL[v] - UR[v])));
-            }
+            for (int v = 0; v < N; v++) G[v] = pt < NPT ? q[v] : R(0);
         } else {
-#pragma unroll
-            for (int v = 0; v < N; v++) G[v] = R(0);
+            G[0] = pt < NPT ? gq[0] : R(0);
         }
         const V Asp = tabA[MFMA_A_SP * 64 + lane];
 #pragma unroll

@@ -6,8 +6,10 @@
 // on the device (geometry, tiling, initial condition, FP64 audit of the
 // initial state) on the solve's own stream, and hands it to the GPU thread.
 // The GPU thread admits ready solves into the batch engine between rounds,
-// replays `steps_per_round` captured batch steps (every resident solve takes
-// its own CFL step per batch step), reads the per-solve done flags, and
+// replays `steps_per_round` captured batch steps (every advancing solve
+// takes its own CFL step per batch step; solves of at least fifo_min_elems
+// elements advance one at a time in arrival order, the smaller ones beside
+// them), reads the per-solve done flags, and
 // retires finished solves: FP64 audit of the final state, optional state
 // copy, device memory returned to the budget.  Preparation of new requests
 // overlaps the batch steps of resident ones.
@@ -251,8 +253,31 @@ class ServerImpl : public Server {
         n_finished++;
     }
 
+    // The solves the engine advances: every resident solve below
+    // fifo_min_elems, and of the larger ones (each fills the GPU alone, so
+    // running several together gains no throughput and makes the first to
+    // arrive wait for the others) only the earliest arrival.  Indices into
+    // `active`.
+    std::vector<size_t> stepping_set() const {
+        std::vector<size_t> order(active.size());
+        for (size_t k = 0; k < order.size(); k++) order[k] = k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return active[x].req.arrival_s < active[y].req.arrival_s; });
+        std::vector<size_t> out;
+        bool large_taken = false;
+        for (size_t k : order) {
+            if (cfg.fifo_min_elems > 0 && active[k].s->K >= cfg.fifo_min_elems) {
+                if (large_taken) continue;
+                large_taken = true;
+            }
+            out.push_back(k);
+        }
+        return out;
+    }
+
     void gpu_loop() {
         std::vector<Solver*> resident;
+        std::vector<size_t> stepping;
         std::vector<double> h;
         bool changed = false;
         while (true) {
@@ -271,8 +296,9 @@ class ServerImpl : public Server {
             }
             Status st;
             if (changed) {
+                stepping = stepping_set();
                 resident.clear();
-                for (Job& j : active) resident.push_back(j.s);
+                for (size_t k : stepping) resident.push_back(active[k].s);
                 if (!(st = eng->set_slots(resident))) {
                     fail_all(st.msg);
                     return;
@@ -285,11 +311,15 @@ class ServerImpl : public Server {
                 fail_all(st.msg);
                 return;
             }
+            std::vector<bool> done(active.size(), false);
+            for (size_t r = 0; r < stepping.size(); r++) {
+                const Job& j = active[stepping[r]];
+                done[stepping[r]] = h[3 * r] != 0.0 || !(h[3 * r + 2] < j.req.t_end);
+            }
             std::vector<Job> still;
             for (size_t k = 0; k < active.size(); k++) {
                 Job& j = active[k];
-                const bool done = h[3 * k] != 0.0 || !(h[3 * k + 2] < j.req.t_end);
-                if (!done) {
+                if (!done[k]) {
                     still.push_back(j);
                     continue;
                 }
@@ -327,7 +357,7 @@ Status Server::create(const dg_server_config& cfg, Server** out) {
     if (cfg.precision != DG_FP32 && cfg.precision != DG_FP64) return Status::err(DG_ERR_ARG, "bad precision");
     if (cfg.max_slots < 1 || cfg.max_slots > 4096) return Status::err(DG_ERR_ARG, "max_slots must be in 1..4096");
     if (cfg.hbm_budget <= 0 || cfg.steps_per_round < 1 || !(cfg.cfl > 0) || cfg.solo_min_elems < 0 ||
-        cfg.prep_threads < 0 || cfg.prep_threads > 64)
+        cfg.prep_threads < 0 || cfg.prep_threads > 64 || cfg.fifo_min_elems < 0)
         return Status::err(DG_ERR_ARG, "bad config");
     auto* s = new ServerImpl();
     s->cfg = cfg;

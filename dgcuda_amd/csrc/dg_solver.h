@@ -66,7 +66,7 @@ struct LaunchShape {
 struct SlotInfo {
     void *c, *acc, *ua, *ub, *geo_e, *geo_f;
     const int* nbr;
-    const uint16_t* fword;
+    const uint32_t* fword;
     const int* halo;
     const void* face4;      // packed geometry of the matrix-core kernels
     const void* metric4;

@@ -291,7 +291,7 @@ class SolverT : public Solver {
     int* perm_d = nullptr;  // [Kp]: caller's element of internal element i (padding: element K-1's)
     int device = -1;
     std::vector<int> h_nbr[3];
-    uint16_t* fword = nullptr;
+    uint32_t* fword = nullptr;
     int* halo = nullptr;
     void *face4 = nullptr, *metric4 = nullptr;
     int64_t tile_bytes = 0;  // fword, halo and packed geometry
@@ -443,36 +443,46 @@ class SolverT : public Solver {
     }
 
     // Matrix-core tiling: per (element, face) the LDS slot of the neighbour
-    // (inside the tile, or in the tile's halo list), and the halo lists.
+    // (inside the tile, or in the tile's halo list), the halo lists, and the
+    // tile's side list: the sides the tile computes (boundary and halo sides,
+    // and of each side between two of its elements the left element's
+    // record), numbered in record order.
     Status build_mfma_tiles(int tile_elems) {
         const int ntiles = (K + tile_elems - 1) / tile_elems;
-        std::vector<uint16_t> words(3 * (size_t)Kp, 0);
+        std::vector<uint32_t> words(3 * (size_t)Kp, 0);
         std::vector<std::vector<int>> halos(ntiles);
         int hmax = 0;
         for (int t = 0; t < ntiles; t++) {
             const int b = t * tile_elems, end = std::min(b + tile_elems, K);
             std::vector<int>& h = halos[t];
+            uint32_t ntask = 0;
             for (int e = b; e < end; e++)
                 for (int f = 0; f < 3; f++) {
                     const int wv = h_nbr[f][e];
+                    uint32_t wd;
+                    bool owner;
                     if (wv < 0) {
                         // boundary side: own slot and face, left, flag bit 15
                         // (the kernel reads the ghost data through the
                         // neighbour word)
-                        words[f * (size_t)Kp + e] = (uint16_t)((e - b) | (f << 12) | (1 << 14) | (1 << 15));
-                        continue;
-                    }
-                    const int nb = wv & NB_ELEM_MASK, fn = (wv >> 28) & 3, left = (wv >> 30) & 1;
-                    int slot;
-                    if (nb >= b && nb < end) {
-                        slot = nb - b;
+                        wd = (uint32_t)((e - b) | (f << 12) | (1 << 14) | (1 << 15));
+                        owner = true;
                     } else {
-                        auto it = std::find(h.begin(), h.end(), nb);
-                        slot = tile_elems + (int)(it - h.begin());
-                        if (it == h.end()) h.push_back(nb);
+                        const int nb = wv & NB_ELEM_MASK, fn = (wv >> 28) & 3, left = (wv >> 30) & 1;
+                        int slot;
+                        if (nb >= b && nb < end) {
+                            slot = nb - b;
+                        } else {
+                            auto it = std::find(h.begin(), h.end(), nb);
+                            slot = tile_elems + (int)(it - h.begin());
+                            if (it == h.end()) h.push_back(nb);
+                        }
+                        if (slot >= 4096) return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
+                        wd = (uint32_t)(slot | (fn << 12) | (left << 14));
+                        owner = slot >= tile_elems || left;
                     }
-                    if (slot >= 4096) return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
-                    words[f * (size_t)Kp + e] = (uint16_t)(slot | (fn << 12) | (left << 14));
+                    if (owner) wd |= (ntask++ << FW_TASK_SHIFT) | FW_OWNER;
+                    words[f * (size_t)Kp + e] = wd;
                 }
             hmax = std::max(hmax, (int)h.size());
         }
@@ -491,7 +501,7 @@ class SolverT : public Solver {
         face4 = nullptr;
         metric4 = nullptr;
         tile_bytes = 0;
-        const size_t bw = words.size() * sizeof(uint16_t), bh = hl.size() * sizeof(int);
+        const size_t bw = words.size() * sizeof(uint32_t), bh = hl.size() * sizeof(int);
         const size_t bf = (size_t)Kp * 3 * 4 * sizeof(R), bm = (size_t)Kp * 4 * sizeof(R);
         Status sa = pool_alloc((void**)&fword, bw);
         if (sa) sa = pool_alloc((void**)&halo, bh);

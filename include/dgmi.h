@@ -366,6 +366,9 @@ typedef struct {
     int solo_min_elems;     /* solves with at least this many elements run their own stage
                                kernels inside the step graph (0: all in the batch launch) */
     int prep_threads;       /* host threads preparing admitted solves (mesh, setup, projection) */
+    int fifo_min_elems;     /* solves with at least this many elements advance one at a time, in
+                               arrival order, the smaller ones beside them (0: every resident
+                               solve advances at every step) */
 } dg_server_config;
 
 typedef struct {

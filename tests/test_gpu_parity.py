@@ -112,6 +112,35 @@ def test_c2_vortex_coarse_full_run():
         assert err <= tol
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_integrate_continues(kernel):
+    """A run to t_end followed by another run (to a later t_end, or a number
+    of steps) starts from the wave speed of the state the first run left:
+    FP64 within 1e-12 of the oracle doing the same, with the same step
+    counts.  The steps a run enqueues past t_end are not taken and must not
+    consume that wave speed (they once did: the next run's first step
+    divided by a zero maximum and jumped to its end time)."""
+    m = mesh(16)
+    prob = problem_for(4)
+    o = orc.Oracle.from_mesh(m, 2, prob)
+    u0 = o.project_ic(orc.ic_array(prob, m.period))
+    u1, n1, _ = o.integrate(u0, 0.1, 0.3)
+    u2, n2, _ = o.integrate(u1, 0.1, 0.3)
+    u3, _ = o.run_steps(u2, 3, 0.3)
+    s = dg.Solver(m, prob, 2, dg.DG_FP64)
+    s.set_kernel(kernel)
+    s.project_ic()
+    g1, t1 = s.integrate(0.1, 0.3)
+    e1 = orc.rel_l2(s.get_state(), u1)
+    g2, t2 = s.integrate(0.2, 0.3)
+    e2 = orc.rel_l2(s.get_state(), u2)
+    s.run_steps(3, 0.3)
+    e3 = orc.rel_l2(s.get_state(), u3)
+    print("kernel %d steps %d+%d (oracle %d+%d) rel L2 %.2e %.2e %.2e" % (kernel, g1, g2, n1, n2, e1, e2, e3))
+    assert (g1, g2) == (n1, n2) and t1 == 0.1 and t2 == 0.2
+    assert e1 <= 1e-12 and e2 <= 1e-12 and e3 <= 1e-12
+
+
 def test_graph_replay_bitwise():
     m = mesh(16)
     prob = problem_for(4)

@@ -29,7 +29,8 @@ def requests(n, seed=3):
 
 def alone(req, solo_min_elems):
     """The same request served by itself."""
-    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems)
+    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems,
+                   fifo_min_elems=0)
     sv.keep_states(True)
     r = dg.request(req.nx, req.p, req.t_end, req.prob, mesh_seed=req.mesh_seed, arrival_s=0.0)
     i = sv.submit(r)
@@ -41,14 +42,18 @@ def alone(req, solo_min_elems):
     return state
 
 
-@pytest.mark.parametrize("solo_min_elems", [0, 400])
-def test_server_results_equal_alone(solo_min_elems):
+@pytest.mark.parametrize("solo_min_elems,fifo_min_elems", [(0, 0), (400, 0), (400, 600)])
+def test_server_results_equal_alone(solo_min_elems, fifo_min_elems):
     """In the mix, every result equals the request served alone; with a
     threshold of 400 elements the larger requests run their own kernels
-    inside the step graph while the others share the batch launch."""
+    inside the step graph while the others share the batch launch; with a
+    FIFO threshold of 600 the requests of 600 elements and more advance one
+    at a time in arrival order, the others beside them."""
     reqs = requests(10)
     assert any(2 * r.nx * r.ny >= 400 for r in reqs) and any(2 * r.nx * r.ny < 400 for r in reqs)
-    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems)
+    assert sum(2 * r.nx * r.ny >= 600 for r in reqs) >= 2
+    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems,
+                   fifo_min_elems=fifo_min_elems)
     sv.keep_states(True)
     t0 = sv.now()
     ids = []
@@ -69,6 +74,11 @@ def test_server_results_equal_alone(solo_min_elems):
         assert out.finish_s >= out.arrival_s
         m0, m1 = np.array(out.mass0), np.array(out.mass)
         assert np.all(np.abs(m1 - m0) <= 2e-6 * np.abs(m0) + 1e-9)
+    if fifo_min_elems:
+        # the large requests finished in arrival order, each before the next
+        # one took a step: admission order is arrival order here
+        large = [res[ids[k]] for k in range(len(reqs)) if 2 * reqs[k].nx * reqs[k].ny >= fifo_min_elems]
+        assert [x.finish_s for x in large] == sorted(x.finish_s for x in large)
 
 
 def test_server_solo_turnover():
