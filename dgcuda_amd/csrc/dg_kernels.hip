@@ -21,8 +21,23 @@ namespace dgmi {
 
 DGMI_TABLE_UPLOADER(upload_tables_kernels)
 
+// Occupancy the register allocation of the lane kernel is held to at p = 1
+// and p = 2 (wavefronts per SIMD; 1 leaves the compiler free).  Each lane
+// gathers its neighbours' modes; more wavefronts in flight cover more of
+// that latency.
+#ifndef DGMI_VALU_MINW_P1
+#define DGMI_VALU_MINW_P1 1
+#endif
+#ifndef DGMI_VALU_MINW_P2
+#define DGMI_VALU_MINW_P2 1
+#endif
+template <int P>
+constexpr int valu_minw() {
+    return P == 1 ? DGMI_VALU_MINW_P1 : P == 2 ? DGMI_VALU_MINW_P2 : 1;
+}
+
 template <int N, int P, typename R>
-__global__ void __launch_bounds__(128) k_stage_valu(StageArgs<R> a) {
+__global__ void __launch_bounds__(128, valu_minw<P>()) k_stage_valu(StageArgs<R> a) {
     __shared__ double sh[8];
     if (!a.ctrl->active) return;
     const int tile = tile_of_block(blockIdx.x, gridDim.x, a.xcd_remap);
