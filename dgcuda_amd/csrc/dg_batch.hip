@@ -399,9 +399,13 @@ class EngineT : public BatchEngine {
             SlotInfo si;
             solvers[s]->slot_info(&si);
             const bool tiles = si.kind >= 2;
-            if (tiles != (si.p >= 3))
+            // solves with boundary sides always run their own kernels (the
+            // batched bodies carry no boundary code)
+            const bool solo_here = si.boundary || (solo_min > 0 && si.K >= solo_min);
+            if (!solo_here && tiles != (si.p >= 3))
                 return Status::err(DG_ERR_STATE, "batch slots use matrix-core tiles exactly when p >= 3");
-            if (tiles && si.hcap > MFMA_MAX_HALO) return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
+            if (!solo_here && tiles && si.hcap > MFMA_MAX_HALO)
+                return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
             SlotDesc<R>& d = hs[s];
             d.c = (R*)si.c;
             d.acc = (R*)si.acc;
@@ -422,7 +426,7 @@ class EngineT : public BatchEngine {
             d.hcap = si.hcap;
             d.path = tiles ? 2 : 1;
             d.phys = si.phys;
-            if (solo_min > 0 && si.K >= solo_min) {
+            if (solo_here) {
                 new_solo.push_back(solvers[s]);  // slot kept for dt and status; no work items
                 continue;
             }
@@ -597,12 +601,15 @@ Status BatchEngine::create(int precision, int max_slots, int max_items, BatchEng
 
 Status make_batch_solver(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out) {
     if (p < 1 || p > 4) return Status::err(DG_ERR_UNSUPPORTED, "batched solves support p = 1..4");
-    for (int s = 0; s < m.S; s++)
-        if (m.right_elem[s] < 0)
-            return Status::err(DG_ERR_UNSUPPORTED, "batched solves need meshes without boundary sides");
-    // the batch's fixed per-order choice (no autotuning launches at setup)
+    bool boundary = false;
+    for (int s = 0; s < m.S && !boundary; s++) boundary = m.right_elem[s] < 0;
+    // the batch's fixed per-order choice (no autotuning launches at setup);
+    // a solve with boundary sides runs its own kernels inside the batch step:
+    // the matrix-core kernel for Euler at p >= 3, else the lane kernel (the
+    // solver refuses the orders neither supports)
+    const int kind = boundary ? (p >= 3 && prob.physics == 4 ? 2 : 1) : (p >= 3 ? 2 : 1);
     Solver* s = nullptr;
-    Status st = Solver::create(m, prob, p, precision, &s, p >= 3 ? 2 : 1);
+    Status st = Solver::create(m, prob, p, precision, &s, kind);
     if (!st) return st;
     *out = s;
     return Status::ok();

@@ -72,6 +72,7 @@ struct SlotInfo {
     const void* metric4;
     Ctrl* ctrl;
     int K, Kp, N, p, hcap, kind;
+    int boundary;           // the mesh has boundary sides (batched as a solo solve)
     PhysP phys;
 };
 

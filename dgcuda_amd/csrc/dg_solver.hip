@@ -1037,6 +1037,7 @@ class SolverT : public Solver {
         o->p = p;
         o->hcap = mx.hcap;
         o->kind = shape.kind;
+        o->boundary = has_boundary ? 1 : 0;
         o->phys = phys;
     }
 

@@ -49,7 +49,7 @@ typedef enum {
     DG_ERR_IO = 2,          /* file could not be read or written */
     DG_ERR_PARSE = 3,       /* malformed mesh file */
     DG_ERR_HIP = 4,         /* HIP runtime error or no GPU */
-    DG_ERR_UNSUPPORTED = 5, /* valid but not supported (e.g. a boundary mesh in a batch) */
+    DG_ERR_UNSUPPORTED = 5, /* valid but not supported (e.g. p = 5 in a batch) */
     DG_ERR_NUMERIC = 6,     /* non-finite state or non-positive density / pressure */
     DG_ERR_BUDGET = 7,      /* request exceeds the memory budget */
     DG_ERR_STATE = 8        /* call out of order */
@@ -332,7 +332,11 @@ int dg_selftest_mfma(double* max_err);
 /* ---------------- many solves, one persistent launch per RK stage ---------------- */
 int dg_batch_create(int precision, dg_batch** out);
 void dg_batch_destroy(dg_batch* b);
-/* adds one solve (its own mesh, problem, order p, end time); returns its slot id */
+/* adds one solve (its own mesh, problem, order p = 1..4, end time); returns
+ * its slot id.  A mesh with boundary sides runs its own stage kernels inside
+ * the batch's captured step (the batched work items carry no boundary
+ * code): Euler at p >= 3 on the matrix-core kernel, other orders on the lane
+ * kernel, which takes boundary sides up to p = 3. */
 int dg_batch_add(dg_batch* b, const dg_mesh* m, const dg_problem_desc* prob, int p, double t_end, int* slot);
 /* projects the initial conditions of every solve and resets their clocks */
 int dg_batch_project_ic(dg_batch* b);

@@ -7,7 +7,8 @@ over a run with the same step count; FP32 within 1e-4 per RHS (a
 near-steady state: k is a small difference of O(1) fluxes) and 1e-5 over a
 run.  Boundary meshes run on the lane-per-element kernel up to p = 3 and,
 for Euler, on the matrix-core kernels (kinds 2 and 3, their boundary
-variant) up to p = 4; batched solves refuse them.  The first run of the
+variant) up to p = 4; in a batch such a solve runs its own kernels inside
+the batch's captured step.  The first run of the
 p = 4 FP64 lane kernel with the boundary path faulted the GPU (memory
 aperture violation) and its cause is not found: the lane kernel refuses
 boundary meshes at p >= 4 (checked here and on the CPU in test_abi) and no
@@ -107,10 +108,52 @@ def test_boundary_mesh_kernel_choice():
     with pytest.raises(dg.DGError) as e:
         s4.set_kernel(1)  # the lane kernel takes boundary sides up to p = 3
     assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
+    # a batch takes boundary meshes as solves with their own kernels, except
+    # where no kernel takes them (advection at p = 4)
     b = dg.Batch(dg.DG_FP32)
+    b.add(m, prob, 2, 0.1)
     with pytest.raises(dg.DGError) as e:
-        b.add(m, prob, 2, 0.1)
+        b.add(m, dg.advection_gaussian(), 4, 0.1)
     assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [dg.DG_FP64, dg.DG_FP32])
+def test_batch_with_boundary_solves(prec):
+    """Supersonic vortices on the reference's sv1refined mesh (p = 2 on the
+    lane kernel, p = 3 and 4 on the matrix-core kernel's boundary variant)
+    batched with periodic solves: each boundary solve equals the same solve
+    alone on that kernel bit for bit, the periodic ones are unaffected, and
+    FP64 matches the oracle to 1e-12 with the same step count."""
+    m = golden_mesh("sv1refined")
+    prob = dg.supersonic_vortex()
+    per = dg.Mesh.periodic_jittered(10, 10, 0.0, 10.0, 0.0, 10.0, 0.1, 3)
+    vort = dg.euler_vortex(((4.0, 5.0, 5.0),))
+    cases = [(m, prob, 2), (per, vort, 3), (m, prob, 3), (per, vort, 1), (m, prob, 4)]
+    b = dg.Batch(prec)
+    for mm, pr, p in cases:
+        b.add(mm, pr, p, 0.03)
+    b.project_ic()
+    b.integrate(0.3)
+    for k, (mm, pr, p) in enumerate(cases):
+        s = dg.Solver(mm, pr, p, prec)
+        s.set_kernel(2 if p >= 3 else 1)
+        s.project_ic()
+        n, _ = s.integrate(0.03, 0.3)
+        got = b.get_state(k)
+        assert b.audit(k)["t"] == 0.03 and b.audit(k)["steps"] == n
+        if mm is m or p >= 3:
+            assert np.array_equal(got, s.get_state()), k
+        else:
+            assert rel_max(got, s.get_state()) <= (1e-6 if prec == dg.DG_FP32 else 1e-14), k
+        s.close()
+        if prec == dg.DG_FP64 and mm is m:
+            o = orc.Oracle.from_mesh(mm, p, pr)
+            u0 = o.project_ic(orc.ic_array(pr, mm.period))
+            ref, n_ref, _ = o.integrate(u0, 0.03, 0.3)
+            err = orc.rel_l2(got, ref)
+            print("slot %d p=%d steps %d/%d rel L2 vs oracle %.3e" % (k, p, n, n_ref, err))
+            assert n == n_ref and err <= 1e-12
     b.close()
 
 
