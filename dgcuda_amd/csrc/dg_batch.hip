@@ -601,13 +601,10 @@ Status BatchEngine::create(int precision, int max_slots, int max_items, BatchEng
 
 Status make_batch_solver(const Mesh& m, const dg_problem_desc& prob, int p, int precision, Solver** out) {
     if (p < 1 || p > 4) return Status::err(DG_ERR_UNSUPPORTED, "batched solves support p = 1..4");
-    bool boundary = false;
-    for (int s = 0; s < m.S && !boundary; s++) boundary = m.right_elem[s] < 0;
     // the batch's fixed per-order choice (no autotuning launches at setup);
-    // a solve with boundary sides runs its own kernels inside the batch step:
-    // the matrix-core kernel for Euler at p >= 3, else the lane kernel (the
-    // solver refuses the orders neither supports)
-    const int kind = boundary ? (p >= 3 && prob.physics == 4 ? 2 : 1) : (p >= 3 ? 2 : 1);
+    // a solve with boundary sides runs its own kernels (the same choice)
+    // inside the batch step
+    const int kind = p >= 3 ? 2 : 1;
     Solver* s = nullptr;
     Status st = Solver::create(m, prob, p, precision, &s, kind);
     if (!st) return st;

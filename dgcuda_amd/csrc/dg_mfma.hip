@@ -171,14 +171,10 @@ template hipError_t launch_mfma_pack<double>(const double*, const double*, const
 // workgroups of the matrix-core kernel that fit on the device at once
 template <typename R, bool BND>
 static const void* stage2_kernel(int N, int p) {
-    if constexpr (BND) {
-        if (N != 4) return nullptr;
-    } else {
-        if (N == 1 && p == 4) return reinterpret_cast<const void*>(&k_stage_mfma<1, 4, ST_RK4_2, R>);
-        if (N == 1 && p == 3) return reinterpret_cast<const void*>(&k_stage_mfma<1, 3, ST_RK4_2, R>);
-        if (N == 1 && p == 2) return reinterpret_cast<const void*>(&k_stage_mfma<1, 2, ST_RK4_2, R>);
-        if (N == 1 && p == 1) return reinterpret_cast<const void*>(&k_stage_mfma<1, 1, ST_RK4_2, R>);
-    }
+    if (N == 1 && p == 4) return reinterpret_cast<const void*>(&k_stage_mfma<1, 4, ST_RK4_2, R, BND>);
+    if (N == 1 && p == 3) return reinterpret_cast<const void*>(&k_stage_mfma<1, 3, ST_RK4_2, R, BND>);
+    if (N == 1 && p == 2) return reinterpret_cast<const void*>(&k_stage_mfma<1, 2, ST_RK4_2, R, BND>);
+    if (N == 1 && p == 1) return reinterpret_cast<const void*>(&k_stage_mfma<1, 1, ST_RK4_2, R, BND>);
     if (N == 4 && p == 4) return reinterpret_cast<const void*>(&k_stage_mfma<4, 4, ST_RK4_2, R, BND>);
     if (N == 4 && p == 3) return reinterpret_cast<const void*>(&k_stage_mfma<4, 3, ST_RK4_2, R, BND>);
     if (N == 4 && p == 2) return reinterpret_cast<const void*>(&k_stage_mfma<4, 2, ST_RK4_2, R, BND>);
@@ -250,7 +246,7 @@ static hipError_t launch_np(const StageArgs<R>& a, const MfmaArgs& mx, dim3 grid
 }
 
 // a.bnd set: the mesh has boundary sides and the launch takes the BND
-// variant (Euler only; the host refuses advection on such meshes here)
+// variant
 template <typename R>
 hipError_t launch_stage_mfma(int N, int p, const StageArgs<R>& a, const MfmaArgs& mx, dim3 grid, hipStream_t st) {
     if (a.bnd) {
@@ -258,6 +254,10 @@ hipError_t launch_stage_mfma(int N, int p, const StageArgs<R>& a, const MfmaArgs
         if (N == 4 && p == 3) return launch_np<4, 3, R, true>(a, mx, grid, st);
         if (N == 4 && p == 2) return launch_np<4, 2, R, true>(a, mx, grid, st);
         if (N == 4 && p == 1) return launch_np<4, 1, R, true>(a, mx, grid, st);
+        if (N == 1 && p == 4) return launch_np<1, 4, R, true>(a, mx, grid, st);
+        if (N == 1 && p == 3) return launch_np<1, 3, R, true>(a, mx, grid, st);
+        if (N == 1 && p == 2) return launch_np<1, 2, R, true>(a, mx, grid, st);
+        if (N == 1 && p == 1) return launch_np<1, 1, R, true>(a, mx, grid, st);
         return hipErrorInvalidValue;
     }
     if (N == 4 && p == 4) return launch_np<4, 4, R>(a, mx, grid, st);

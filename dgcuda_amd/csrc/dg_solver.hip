@@ -298,7 +298,7 @@ class SolverT : public Solver {
     MfmaArgs mx{};
     double *V = nullptr, *Jd = nullptr, *audit_part = nullptr;
     double* bnd = nullptr;      // boundary sides' ghost data (boundary_data)
-    bool has_boundary = false;  // the lane kernel only
+    bool has_boundary = false;  // the mesh has boundary sides (bnd holds their ghost data)
     Ctrl* ctrl = nullptr;
     Ctrl hctrl{};
     LaunchShape shape;
@@ -368,10 +368,10 @@ class SolverT : public Solver {
         std::string why = elem_geometry(m, perm, pr.quirks, g);
         if (!why.empty()) return Status::err(DG_ERR_UNSUPPORTED, why);
         has_boundary = !g.bfaces.empty();
-        if (has_boundary && p > VALU_BOUNDARY_MAX_P && !(N == 4 && mfma_supported(N, p)))
+        if (has_boundary && p > VALU_BOUNDARY_MAX_P && !mfma_supported(N, p))
             return Status::err(DG_ERR_UNSUPPORTED,
-                               "boundary sides are supported up to p = 3 (lane kernel) and for Euler up to p = 4 "
-                               "(matrix-core kernels)");
+                               "boundary sides are supported up to p = 4 (lane kernel up to p = 3, matrix-core "
+                               "kernels up to p = 4)");
         min_r = g.min_r;
         Status st = ensure_device_ready();
         if (!st) return st;
@@ -527,8 +527,6 @@ class SolverT : public Solver {
         const bool mf = mfma_supported(N, p);
         if ((kind == 2 || kind == 3) && !mf)
             return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel needs p = 1..4");
-        if ((kind == 2 || kind == 3) && has_boundary && N != 4)
-            return Status::err(DG_ERR_UNSUPPORTED, "matrix-core kernel with boundary sides needs Euler");
         if (kind == 1 && has_boundary && p > VALU_BOUNDARY_MAX_P)
             return Status::err(DG_ERR_UNSUPPORTED, "lane kernel with boundary sides needs p <= 3");
         if (kind < 1 || kind > 3) return Status::err(DG_ERR_ARG, "kernel kind must be 0, 1, 2 or 3");

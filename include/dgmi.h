@@ -230,8 +230,8 @@ int dg_tables(int p, int* sizes, double* r1, double* r2, double* w, double* s_r,
  * (euler.cu:177-241), reflecting the interior density and energy and the
  * velocity's component along the wall whose normal is the radial direction
  * (supersonic.cu:64-108).  Such meshes run on the lane-per-element kernel
- * up to p = 3 and, for Euler, on the matrix-core kernels up to p = 4; other
- * orders and physics on them return DG_ERR_UNSUPPORTED, as do batches. */
+ * up to p = 3 and on the matrix-core kernels up to p = 4 (advection and
+ * Euler); p = 5 on them returns DG_ERR_UNSUPPORTED. */
 int dg_solver_create(const dg_mesh* m, const dg_problem_desc* prob, int p, int precision, dg_solver** out);
 void dg_solver_destroy(dg_solver* s);
 /* sizes[0..5] = num_elem, num_sides, n_p, n_q, n_q1, N */
@@ -335,8 +335,8 @@ void dg_batch_destroy(dg_batch* b);
 /* adds one solve (its own mesh, problem, order p = 1..4, end time); returns
  * its slot id.  A mesh with boundary sides runs its own stage kernels inside
  * the batch's captured step (the batched work items carry no boundary
- * code): Euler at p >= 3 on the matrix-core kernel, other orders on the lane
- * kernel, which takes boundary sides up to p = 3. */
+ * code): p >= 3 on the matrix-core kernel's boundary variant, p <= 2 on the
+ * lane kernel. */
 int dg_batch_add(dg_batch* b, const dg_mesh* m, const dg_problem_desc* prob, int p, double t_end, int* slot);
 /* projects the initial conditions of every solve and resets their clocks */
 int dg_batch_project_ic(dg_batch* b);

@@ -142,17 +142,15 @@ def test_read_reference_boundary_mesh():
     with pytest.raises(dg.DGError) as e:
         dg.Solver(m, dg.supersonic_vortex(), 1)
     assert e.value.code == dg.STATUS_CODES["DG_ERR_HIP"]
-    # boundary sides: p <= 3 on the lane kernel, Euler p <= 4 on the
-    # matrix-core kernels; beyond that refused before any device work
-    with pytest.raises(dg.DGError) as e:
-        dg.Solver(m, dg.supersonic_vortex(), 4)
-    assert e.value.code == dg.STATUS_CODES["DG_ERR_HIP"]
-    with pytest.raises(dg.DGError) as e:
-        dg.Solver(m, dg.supersonic_vortex(), 5)
-    assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
-    with pytest.raises(dg.DGError) as e:
-        dg.Solver(m, dg.advection_gaussian(), 4)
-    assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
+    # boundary sides: p <= 3 on the lane kernel, p <= 4 on the matrix-core
+    # kernels (Euler and advection); p = 5 refused before any device work
+    for prob in (dg.supersonic_vortex(), dg.advection_gaussian()):
+        with pytest.raises(dg.DGError) as e:
+            dg.Solver(m, prob, 4)
+        assert e.value.code == dg.STATUS_CODES["DG_ERR_HIP"]
+        with pytest.raises(dg.DGError) as e:
+            dg.Solver(m, prob, 5)
+        assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
 
 
 def test_read_errors(tmp_path):

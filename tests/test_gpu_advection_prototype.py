@@ -1,7 +1,7 @@
 """The advection prototype's path on the GPU against the FP64 oracle: the
 central flux (every kernel, both precisions), its exact-boundary meshes
-read with their sides built at read time (lane kernel, which carries
-boundary sides for advection up to p = 3), fixed-step RK4 as the reference's
+read with their sides built at read time (lane kernel up to p = 3,
+matrix-core kernels up to p = 4), fixed-step RK4 as the reference's
 time-step test runs it, and a central-flux solve in a batch.
 
 Bars: FP64 1e-12 relative (RHS max-norm, runs L2), FP32 2e-5 per RHS and
@@ -69,21 +69,26 @@ def test_central_flux_run_and_batch(prec):
 def test_exact_boundary_time_steps_vs_oracle(name, dt, prec):
     """(x - y)^alpha on the prototype's meshes, every boundary side exact,
     10 RK4 steps of the test's dt; alpha > p too (not in the space, so the
-    state moves)."""
+    state moves).  The lane kernel up to p = 3, the matrix-core kernels'
+    boundary variant (kinds 2 and 3) up to p = 4."""
     m = dg.Mesh.read_vertex_pmsh(os.path.join(GOLD_DIR, "t0", name + ".pmsh"))
-    for p in (1, 2, 3):
+    for p in (1, 2, 3, 4):
         for alpha in (p, p + 1):
-            print("exact boundary %s p=%d alpha=%d prec=%d" % (name, p, alpha, prec), flush=True)
             prob = dg.advection_poly(alpha)
             o = orc.Oracle.from_mesh(m, p, prob)
-            u = o.project_ic(orc.ic_array(prob, m.period))
-            s = dg.Solver(m, prob, p, prec)
-            s.set_kernel(1)
-            s.set_state(u)
+            u0 = o.project_ic(orc.ic_array(prob, m.period))
+            ref = u0
             for _ in range(10):
-                u = o.rk_step(u, dt, 1)
-                s.rk_step(dt)
-            err = orc.rel_l2(s.get_state(), u)
-            assert err <= (1e-12 if prec == dg.DG_FP64 else 1e-5), err
-            assert np.abs(s.eval_vertices(0) - o.eval_vertices(u, 0)).max() <= (
-                1e-12 if prec == dg.DG_FP64 else 1e-5) * max(1.0, np.abs(u).max())
+                ref = o.rk_step(ref, dt, 1)
+            for kind in ((1, 2, 3) if p <= 3 else (2, 3)):
+                print("exact boundary %s p=%d alpha=%d prec=%d kernel %d" % (name, p, alpha, prec, kind), flush=True)
+                s = dg.Solver(m, prob, p, prec)
+                s.set_kernel(kind)
+                s.set_state(u0)
+                for _ in range(10):
+                    s.rk_step(dt)
+                err = orc.rel_l2(s.get_state(), ref)
+                assert err <= (1e-12 if prec == dg.DG_FP64 else 1e-5), err
+                assert np.abs(s.eval_vertices(0) - o.eval_vertices(ref, 0)).max() <= (
+                    1e-12 if prec == dg.DG_FP64 else 1e-5) * max(1.0, np.abs(ref).max())
+                s.close()

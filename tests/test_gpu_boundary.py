@@ -6,8 +6,8 @@ Bars: FP64 within 1e-13 relative max-norm per RHS and 1e-12 relative L2
 over a run with the same step count; FP32 within 1e-4 per RHS (a
 near-steady state: k is a small difference of O(1) fluxes) and 1e-5 over a
 run.  Boundary meshes run on the lane-per-element kernel up to p = 3 and,
-for Euler, on the matrix-core kernels (kinds 2 and 3, their boundary
-variant) up to p = 4; in a batch such a solve runs its own kernels inside
+on the matrix-core kernels (kinds 2 and 3, their boundary variant, Euler
+and advection) up to p = 4; in a batch such a solve runs its own kernels inside
 the batch's captured step.  The first run of the
 p = 4 FP64 lane kernel with the boundary path faulted the GPU (memory
 aperture violation) and its cause is not found: the lane kernel refuses
@@ -108,12 +108,12 @@ def test_boundary_mesh_kernel_choice():
     with pytest.raises(dg.DGError) as e:
         s4.set_kernel(1)  # the lane kernel takes boundary sides up to p = 3
     assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
-    # a batch takes boundary meshes as solves with their own kernels, except
-    # where no kernel takes them (advection at p = 4)
+    # a batch takes boundary meshes as solves with their own kernels
     b = dg.Batch(dg.DG_FP32)
     b.add(m, prob, 2, 0.1)
+    b.add(m, dg.advection_gaussian(), 4, 0.1)
     with pytest.raises(dg.DGError) as e:
-        b.add(m, dg.advection_gaussian(), 4, 0.1)
+        b.add(m, prob, 5, 0.1)
     assert e.value.code == dg.STATUS_CODES["DG_ERR_UNSUPPORTED"]
     b.close()
 
