@@ -183,23 +183,38 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
                 for (int m = 0; m < NP; m++) gp(a.out)[(size_t)(v * NP + m) * Kp + e] = cj * tot[v][m];
     } else {
+        // The RK operands of each variable are loaded before its first
+        // store: the output can be the RK base itself (the last stage
+        // updates it in place), so a load after a store could not be moved
+        // above it and each would wait for its own round trip to memory.
+        // One round trip per variable instead of one per mode (grouping all
+        // variables made the batched kernel spill).
+        const bool c_in = (kind == ST_RK4_1 || kind == ST_RK2_1);
+        const bool acc_in = (kind == ST_RK4_2 || kind == ST_RK4_3 || kind == ST_RK4_4);
         R c0[N];
 #pragma unroll
-        for (int v = 0; v < N; v++)
+        for (int v = 0; v < N; v++) {
+            R cv[NP], av[NP];
+#pragma unroll
+            for (int m = 0; m < NP; m++) {
+                const size_t ix = (size_t)(v * NP + m) * Kp + ee;
+                cv[m] = c_in ? u[v][m] : gp(a.c)[ix];
+                av[m] = acc_in ? gp(a.acc)[ix] : R(0);
+            }
 #pragma unroll
             for (int m = 0; m < NP; m++) {
                 const size_t ix = (size_t)(v * NP + m) * Kp + ee;
                 const R k = cj * tot[v][m];
-                const R c = (kind == ST_RK4_1 || kind == ST_RK2_1) ? u[v][m] : gp(a.c)[ix];
+                const R c = cv[m];
                 R o;
                 if (kind == ST_RK4_1) {
                     if (valid) gp(a.acc)[ix] = k / R(6);
                     o = c + R(0.5) * k;
                 } else if (kind == ST_RK4_2 || kind == ST_RK4_3) {
-                    if (valid) gp(a.acc)[ix] = gp(a.acc)[ix] + k / R(3);
+                    if (valid) gp(a.acc)[ix] = av[m] + k / R(3);
                     o = (kind == ST_RK4_2) ? c + R(0.5) * k : c + R(1.0) * k;
                 } else if (kind == ST_RK4_4) {
-                    o = c + (gp(a.acc)[ix] + k / R(6));
+                    o = c + (av[m] + k / R(6));
                 } else if (kind == ST_RK2_1) {
                     o = c + R(0.5) * k;
                 } else {
@@ -208,6 +223,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
                 if (valid) gp(a.out)[ix] = o;
                 if (m == 0) c0[v] = o;
             }
+        }
         if (kind == ST_RK4_4 || kind == ST_RK2_2) {
             double U[4];
             const double b0 = tab<double, P>().B[0][0];
