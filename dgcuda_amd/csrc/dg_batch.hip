@@ -227,8 +227,12 @@ __device__ __forceinline__ void batch_tile_items(const SlotDesc<R>* __restrict__
 // loops so the register allocation of one body does not burden the other.
 // Solves use lane items for p <= 2 and tiles for p >= 3 (the kernel choice
 // of a single solve, so results match it bit for bit).
+// Workgroups per CU the FP32 kernel's registers are held to.
+#ifndef DGMI_BATCH_F32_MINW
+#define DGMI_BATCH_F32_MINW 2
+#endif
 template <int KIND>
-__global__ void __launch_bounds__(256, 2) k_batch_stage_f32(const SlotDesc<float>* __restrict__ slots,
+__global__ void __launch_bounds__(256, DGMI_BATCH_F32_MINW) k_batch_stage_f32(const SlotDesc<float>* __restrict__ slots,
                                                            const int2* __restrict__ items,
                                                            const int* __restrict__ parts) {
     extern __shared__ float4 lds4[];

@@ -21,29 +21,15 @@ namespace dgmi {
 
 DGMI_TABLE_UPLOADER(upload_tables_kernels)
 
-// Occupancy the register allocation of the lane kernel is held to at p = 1
-// and p = 2 (wavefronts per SIMD; 1 leaves the compiler free).  Each lane
-// gathers its neighbours' modes; more wavefronts in flight cover more of
-// that latency.
-#ifndef DGMI_VALU_MINW_P1
-#define DGMI_VALU_MINW_P1 1
-#endif
-#ifndef DGMI_VALU_MINW_P2
-#define DGMI_VALU_MINW_P2 1
-#endif
-template <int P>
-constexpr int valu_minw() {
-    return P == 1 ? DGMI_VALU_MINW_P1 : P == 2 ? DGMI_VALU_MINW_P2 : 1;
-}
-
-template <int N, int P, typename R>
-__global__ void __launch_bounds__(128, valu_minw<P>()) k_stage_valu(StageArgs<R> a) {
+// BND: the variant for meshes with boundary sides (p <= VALU_BOUNDARY_MAX_P)
+template <int N, int P, typename R, bool BND = false>
+__global__ void __launch_bounds__(128) k_stage_valu(StageArgs<R> a) {
     __shared__ double sh[8];
     if (!a.ctrl->active) return;
     const int tile = tile_of_block(blockIdx.x, gridDim.x, a.xcd_remap);
     if (tile >= a.ntiles) return;
     const int kind = a.kind;
-    const double lam = valu_process<N, P, R>(a, tile * blockDim.x + threadIdx.x, kind);
+    const double lam = valu_process<N, P, R, BND>(a, tile * blockDim.x + threadIdx.x, kind);
     if (kind == ST_RK4_4 || kind == ST_RK2_2) {
         const double mx = block_max(lam, sh);
         if (threadIdx.x == 0) lam_commit(mx, a.lam_bits);
@@ -174,6 +160,17 @@ __global__ void __launch_bounds__(256) k_audit(const R* c, StateIx ix, const dou
 // ---------------------------------------------------------------------------
 template <int N, typename R>
 static hipError_t launch_stage_valu_np(int p, const StageArgs<R>& a, dim3 grid, hipStream_t st) {
+    if (a.bnd) {
+        static_assert(VALU_BOUNDARY_MAX_P == 3, "boundary instantiations below are p = 0..3");
+        switch (p) {
+            case 0: hipLaunchKernelGGL((k_stage_valu<N, 0, R, true>), grid, dim3(128), 0, st, a); break;
+            case 1: hipLaunchKernelGGL((k_stage_valu<N, 1, R, true>), grid, dim3(128), 0, st, a); break;
+            case 2: hipLaunchKernelGGL((k_stage_valu<N, 2, R, true>), grid, dim3(128), 0, st, a); break;
+            case 3: hipLaunchKernelGGL((k_stage_valu<N, 3, R, true>), grid, dim3(128), 0, st, a); break;
+            default: return hipErrorInvalidValue;  // refused on the host before this
+        }
+        return hipGetLastError();
+    }
     switch (p) {
         case 0: hipLaunchKernelGGL((k_stage_valu<N, 0, R>), grid, dim3(128), 0, st, a); break;
         case 1: hipLaunchKernelGGL((k_stage_valu<N, 1, R>), grid, dim3(128), 0, st, a); break;

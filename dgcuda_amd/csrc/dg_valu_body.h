@@ -31,8 +31,12 @@ __device__ __forceinline__ R face_phi(const TabP<R, P>& T, int f, int m, int j, 
 
 // Processes element e (clamped for loads when invalid, no stores then);
 // returns its cell wave speed after the last stage of a step, else 0.
-template <int N, int P, typename R>
+// BND: the variant for meshes with boundary sides (p <= VALU_BOUNDARY_MAX_P);
+// without it the face loop carries no branch and the loads of the three
+// faces can be scheduled together.
+template <int N, int P, typename R, bool BND = false>
 __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int kind) {
+    static_assert(!BND || P <= VALU_BOUNDARY_MAX_P, "boundary sides are compiled for p <= VALU_BOUNDARY_MAX_P");
     constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1;
     // Euler up to p = 3 uses the normal-flux forms of the matrix-core
     // kernels (fewer operations and registers; p >= 4 keeps its allocation)
@@ -94,10 +98,10 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
     for (int f = 0; f < 3; f++) {
         const int w = gp(a.nbr)[f * Kp + ee];
-        // boundary side (p <= VALU_BOUNDARY_MAX_P): no neighbour, a ghost state
+        // boundary side (BND variant): no neighbour, a ghost state
         bool bnd = false;
         int nb = w & NB_ELEM_MASK;
-        if constexpr (P <= VALU_BOUNDARY_MAX_P) {
+        if constexpr (BND) {
             bnd = w < 0;
             if (bnd) nb = ee;
         }
@@ -130,7 +134,7 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
                     Un[v] += gp(a.u)[(size_t)(v * NP + m) * Kp + nb] * sn;
                 }
             }
-            if constexpr (P <= VALU_BOUNDARY_MAX_P)
+            if constexpr (BND)
                 if (bnd) boundary_ghost<N, NQ1, R>(a.bnd, w, j, Uo, Un);
             if constexpr (NF_FORM) {
                 // Euler: normal fluxes of both sides, the side's sign in

@@ -112,7 +112,8 @@ static int run(const Mesh& m) {
     double lam = 0;
     for (int kind : {ST_RHS, ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4, ST_RK2_1, ST_RK2_2}) {
         a.kind = kind;
-        for (int e = 0; e < threads; e++) lam = std::max(lam, valu_process<N, P, R>(a, e, kind));
+        for (int e = 0; e < threads; e++)
+            lam = std::max(lam, a.bnd ? valu_process<N, P, R, true>(a, e, kind) : valu_process<N, P, R, false>(a, e, kind));
     }
     double sum = 0;
     for (R x : out) sum += (double)x;
