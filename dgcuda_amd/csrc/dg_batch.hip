@@ -227,9 +227,15 @@ __device__ __forceinline__ void batch_tile_items(const SlotDesc<R>* __restrict__
 // loops so the register allocation of one body does not burden the other.
 // Solves use lane items for p <= 2 and tiles for p >= 3 (the kernel choice
 // of a single solve, so results match it bit for bit).
-// Workgroups per CU the FP32 kernel's registers are held to.
+// Workgroups per CU the kernels' registers are held to.  FP32: 2 (3 spills
+// 76-220 B per lane and measured no faster).  FP64: 1, which lets the
+// compiler keep its spills in the accumulation registers (AGPRs) of the
+// single wavefront per SIMD; 2 spills about 1 KB per lane to memory.
 #ifndef DGMI_BATCH_F32_MINW
 #define DGMI_BATCH_F32_MINW 2
+#endif
+#ifndef DGMI_BATCH_F64_MINW
+#define DGMI_BATCH_F64_MINW 1
 #endif
 template <int KIND>
 __global__ void __launch_bounds__(256, DGMI_BATCH_F32_MINW) k_batch_stage_f32(const SlotDesc<float>* __restrict__ slots,
@@ -247,7 +253,7 @@ __global__ void __launch_bounds__(256, DGMI_BATCH_F32_MINW) k_batch_stage_f32(co
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(256, 1) k_batch_stage_f64(const SlotDesc<double>* __restrict__ slots,
+__global__ void __launch_bounds__(256, DGMI_BATCH_F64_MINW) k_batch_stage_f64(const SlotDesc<double>* __restrict__ slots,
                                                            const int2* __restrict__ items,
                                                            const int* __restrict__ parts) {
     extern __shared__ float4 lds4[];
@@ -318,6 +324,7 @@ class EngineT : public BatchEngine {
     int cap_slots = 0, cap_items = 0;
     int nslots = 0, nitems = 0, grid = 0, dt_grid = 0;
     int grid_resident = 0;           // workgroups that fit on the device at once
+    int max_groups_req = 0;          // set_max_groups (0: no cap)
     std::vector<int2> h_items;       // host copy of the work list and its XCD bounds
     std::vector<int> h_parts;
     size_t lds = 0;
@@ -341,7 +348,6 @@ class EngineT : public BatchEngine {
         HIPST(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPST(hipMalloc((void**)&d_xcd, BATCH_PARTS * sizeof(int)));
         HIPST(hipMemsetAsync(d_xcd, 0, BATCH_PARTS * sizeof(int), st));
-        lds = mfma_tile_lds_bytes(4, MFMA_MAX_HALO, (int)sizeof(R));
         if constexpr (sizeof(R) == 8) {  // above 64 KB: raise the kernels' dynamic LDS limit
             int mxl = 0, dv = 0;
             HIPST(hipGetDevice(&dv));
@@ -350,14 +356,31 @@ class EngineT : public BatchEngine {
                                   BatchKernel<R, ST_RK4_3>::fn(), BatchKernel<R, ST_RK4_4>::fn()})
                 HIPST(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, mxl - 16 * 1024));
         }
-        int per_cu = 0, dev = 0, cus = 0;
-        HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BatchKernel<R, ST_RK4_2>::fn(), 256, lds));
-        HIPST(hipGetDevice(&dev));
-        HIPST(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid = grid_resident = std::max(8, per_cu * cus / 8 * 8);
+        Status s2 = shape_for(16);
+        if (!s2) return s2;
         Status s1 = reserve(max_slots, max_items);
         if (!s1) return s1;
         HIPST(hipStreamSynchronize(st));
+        return Status::ok();
+    }
+
+    // Dynamic LDS of the launch for the largest tile halo among the batched
+    // solves (FP64 with the largest halo allowed holds one workgroup per CU,
+    // with the usual halos two), and the persistent grid: the workgroups that
+    // fit at once, or the cap set by set_max_groups.  A change invalidates
+    // the captured step.
+    Status shape_for(int hcap) {
+        const size_t l = mfma_tile_lds_bytes(4, hcap, (int)sizeof(R));
+        int per_cu = 0, dev = 0, cus = 0;
+        HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BatchKernel<R, ST_RK4_2>::fn(), 256, l));
+        HIPST(hipGetDevice(&dev));
+        HIPST(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int res = std::max(8, per_cu * cus / 8 * 8);
+        const int g = max_groups_req == 0 ? res : std::min(res, max_groups_req / 8 * 8);
+        if (l != lds || g != grid) drop_graph();
+        lds = l;
+        grid_resident = res;
+        grid = g;
         return Status::ok();
     }
 
@@ -399,6 +422,7 @@ class EngineT : public BatchEngine {
         std::vector<SlotDesc<R>> hs(ns);
         std::vector<int2> lane, tile;
         std::vector<double> lane_cost, tile_cost;
+        int hmax = 16;  // largest tile halo among the batched solves
         for (int s = 0; s < ns; s++) {
             SlotInfo si;
             solvers[s]->slot_info(&si);
@@ -434,6 +458,7 @@ class EngineT : public BatchEngine {
                 new_solo.push_back(solvers[s]);  // slot kept for dt and status; no work items
                 continue;
             }
+            if (tiles) hmax = std::max(hmax, si.hcap);
             const int te = tiles ? MFMA_TILE : BATCH_VALU_TILE;
             const int nt = (si.K + te - 1) / te;
             const double per_elem = (double)si.N * n_modes(si.p) * sizeof(R);
@@ -444,6 +469,8 @@ class EngineT : public BatchEngine {
                 cost.push_back(per_elem * std::min(te, si.K - t * te));
             }
         }
+        Status sh = shape_for(hmax);
+        if (!sh) return sh;
         // each list in 8 contiguous chunks of equal estimated cost, one per XCD
         std::vector<int> xs(BATCH_PARTS, 0);
         auto split = [&](const std::vector<double>& cost, int base, int off) {
@@ -550,6 +577,7 @@ class EngineT : public BatchEngine {
 
     Status set_max_groups(int n) override {
         if (n < 0 || (n > 0 && n < 8)) return Status::err(DG_ERR_ARG, "max_groups must be 0 or at least 8");
+        max_groups_req = n;
         const int g = n == 0 ? grid_resident : std::min(grid_resident, n / 8 * 8);
         if (g != grid) drop_graph();
         grid = g;
