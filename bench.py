@@ -5,10 +5,11 @@ oracle's throughput beside it.
 
   python bench.py [--gpus 1] [--steps 200] [--warmup 10]
 
-Prints one JSON line.  Beside the headline workload it carries the ragged
-batch ("batch": 64 solves in one launch per RK stage vs one at a time) and
-the serving loop ("serving": served vs plain solves/s and p50/p99 latency
-on seeded Poisson request streams).  A step is one classical RK4 step (4 stages) of the
+Prints one JSON line.  Beside the headline workload it carries C1 and C2
+run to their end times ("configs": steps, time to solution, errors), the
+ragged batch ("batch": 64 solves in one launch per RK stage vs one at a
+time) and the serving loop ("serving": served vs plain solves/s and
+p50/p99 latency on seeded Poisson request streams).  A step is one classical RK4 step (4 stages) of the
 whole mesh with the CFL time step computed on the device; the timed region
 replays the captured step graph `--steps` times with the state resident in
 HBM.
@@ -33,6 +34,14 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, published
 def vortices(seed, n=3, L=10.0):
     rng = np.random.default_rng(seed)
     return tuple((float(rng.uniform(2, 5)), float(rng.uniform(0, L)), float(rng.uniform(0, L))) for _ in range(n))
+
+
+def rel_l2(a, b):
+    """Relative distance of two modal states: sqrt(sum |a - b|^2) /
+    sqrt(sum |b|^2) over every coefficient (Parseval on the orthonormal
+    basis, elements weighted alike)."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.sqrt(np.sum((a - b) ** 2)) / np.sqrt(np.sum(b ** 2)))
 
 
 def algorithmic_bytes_per_elem(N, n_p, s):
@@ -145,6 +154,52 @@ def batch_summary(copy_gbs, steps=20, steps64=4):
     return out
 
 
+def configs_summary():
+    """C1 and C2 on the GPU, to their end time: steps, time to solution and
+    the errors the configurations report.  C1 (advection p = 1, FP64): L2
+    error against the initial pulse after one period.  C2 (vortex p = 2):
+    FP64 and FP32 runs, FP32 against FP64, each against the exact vortex
+    carried to (7, 7), and the mass drift of each."""
+    import dgcuda_amd as dg
+    from dgcuda_amd import workloads as W
+
+    out = {}
+    m, prob, p, t_end = W.advection_c1()
+    s = dg.Solver(m, prob, p, dg.DG_FP64)
+    s.project_ic()
+    u0 = s.get_state()
+    s.sync()
+    t0 = time.perf_counter()
+    n, _ = s.integrate(t_end, 0.3)
+    el = time.perf_counter() - t0
+    out["c1_advection_p1_2048_fp64"] = dict(steps=n, seconds=round(el, 4),
+                                            rel_l2_vs_initial=float("%.3e" % rel_l2(s.get_state(), u0)))
+    s.close()
+    m, prob, p, t_end = W.vortex_c2()
+    exact = dg.Solver(m, dg.euler_vortex(((5.0, 7.0, 7.0),)), p, dg.DG_FP64)
+    exact.project_ic()
+    ue = exact.get_state()
+    exact.close()
+    res = {}
+    for name, prec in (("fp64", dg.DG_FP64), ("fp32", dg.DG_FP32)):
+        s = dg.Solver(m, prob, p, prec)
+        s.project_ic()
+        m0 = np.array(s.audit()["mass"])
+        s.sync()
+        t0 = time.perf_counter()
+        n, _ = s.integrate(t_end, 0.3)
+        el = time.perf_counter() - t0
+        st = s.get_state()
+        drift = float(np.max(np.abs(np.array(s.audit()["mass"]) - m0) / np.abs(m0)))
+        res[name] = (st, dict(steps=n, seconds=round(el, 4), rel_l2_vs_exact=float("%.3e" % rel_l2(st, ue)),
+                              max_rel_mass_drift=float("%.2e" % drift), kernel=s.kernel()))
+        s.close()
+    c2 = {k: v[1] for k, v in res.items()}
+    c2["rel_l2_fp32_vs_fp64"] = float("%.3e" % rel_l2(res["fp32"][0], res["fp64"][0]))
+    out["c2_vortex_p2_32768"] = c2
+    return out
+
+
 def serving_summary(copy_gbs):
     """Served (continuous batching) against plain one-at-a-time solves on the
     same seeded Poisson request streams: the 1k-500k-triangle mix and the
@@ -186,6 +241,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the ragged-batch measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurements")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C1 / C2 runs")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run this many steps (for rocprof) and print nothing else")
     ap.add_argument("--profile-graph", action="store_true",
@@ -272,8 +328,6 @@ def main():
         # CFL steps from the projected state, against FP64 to the same time,
         # and an FP64 run with FP32-rounded geometry (the geometry storage's
         # share of the FP32 error)
-        import oracle as orc
-
         a32 = dg.Solver(mesh, prob, args.p, dg.DG_FP32)
         a32.set_kernel(kernel_kind)
         a32.project_ic()
@@ -293,12 +347,13 @@ def main():
         g64.close()
         extra["accuracy"] = dict(
             steps=args.accuracy_steps, t=t_acc, fp64_steps=n64,
-            rel_l2_fp32_vs_fp64=float("%.3e" % orc.rel_l2(st32, st64)),
-            rel_l2_fp32_geometry_only=float("%.3e" % orc.rel_l2(stg, st64)),
+            rel_l2_fp32_vs_fp64=float("%.3e" % rel_l2(st32, st64)),
+            rel_l2_fp32_geometry_only=float("%.3e" % rel_l2(stg, st64)),
             reference="GPU FP64 run to the same time; the FP64 GPU path matches the FP64 oracle to <=1e-12 in "
                       "tests. rel_l2_fp32_geometry_only: FP64 state and kernels with the geometry rounded "
                       "through FP32, as the FP32 path stores it")
     bw_ms, bw, bw_kernel = dg.bandwidth_probe(1 << 30, 10, with_variant=True)
+    configs = None if args.no_configs else configs_summary()
     batch = None if args.no_batch else batch_summary(bw)
     serving = None if args.no_serving else serving_summary(bw)
     cpu = finish_cpu_baseline(cpu_proc) if cpu_proc is not None else None
@@ -336,6 +391,8 @@ def main():
         "t_reached": t_reached,
         "host": platform.node(),
     }
+    if configs is not None:
+        out["configs"] = configs
     if batch is not None:
         out["batch"] = batch
     if serving is not None:

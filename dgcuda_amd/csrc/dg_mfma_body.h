@@ -415,7 +415,8 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
 
     // ---- phase 1: face traces of the own sub-tile and of this wave's halo pairs ----
     // (points pos(g, 0..3) of every lane are written: a slot holds 16 points
-    // plus a padding row, and points >= 3(p+1) are never read)
+    // plus a padding row; points >= 3(p+1) are read by the surface
+    // projection only to be replaced by zero)
     const V Atr = tabA[MFMA_A_TR * 64 + lane];
     {
         V T[N];
