@@ -34,6 +34,20 @@ def test_binding_compiles_and_reports_missing_gpu(tmp_path, dg):
     assert r.returncode == 1 and "no GPU" in r.stderr
 
 
+def test_integration_notes_show_the_compiled_binding():
+    """The run_dgcuda body the integration notes print is the file this test
+    module compiles (skipped where the notes are not shipped)."""
+    notes = os.path.join(ROOT, "INTEGRATION.md")
+    if not os.path.exists(notes):
+        pytest.skip("INTEGRATION.md not present")
+    with open(notes) as f:
+        text = f.read()
+    m = re.search(r"```c\n(/\* run_dgcuda\.c.*?)```", text, re.S)
+    assert m, "no run_dgcuda.c block in INTEGRATION.md"
+    with open(os.path.join(BIND, "run_dgcuda.c")) as f:
+        assert m.group(1).strip() == f.read().strip()
+
+
 @pytest.mark.gpu
 def test_binding_runs_the_supersonic_vortex(tmp_path):
     exe = build(tmp_path)
