@@ -246,6 +246,8 @@ def main():
                     help="only run this many steps (for rocprof) and print nothing else")
     ap.add_argument("--profile-graph", action="store_true",
                     help="with --profile-steps: replay the captured step graph, as the timed run does")
+    ap.add_argument("--profile-batch", default="", choices=["", "fp32", "fp64"],
+                    help="with --profile-steps: run only the ragged batch (C4) in this precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)" % args.gpus,
@@ -258,6 +260,13 @@ def main():
 
     import dgcuda_amd as dg
     from dgcuda_amd import workloads as W
+
+    if args.profile_steps and args.profile_batch:
+        bprec = dg.DG_FP32 if args.profile_batch == "fp32" else dg.DG_FP64
+        el, work, b = W.run_batch(W.ragged_batch_c4(), args.profile_steps, bprec)
+        b.close()
+        print("profiled %d batch steps in %.4f s" % (args.profile_steps, el), flush=True)
+        return
 
     prec = dg.DG_FP32 if args.precision == "fp32" else dg.DG_FP64
     s_bytes = 4 if prec == dg.DG_FP32 else 8
