@@ -142,15 +142,22 @@ def batch_summary(copy_gbs, steps=20, steps64=4):
     for prec, name, n in ((dg.DG_FP32, "fp32", steps), (dg.DG_FP64, "fp64", steps64)):
         el, work, b = W.run_batch(cases, n, prec)
         b.close()
+        elc, _, b = W.run_batch(cases, n, prec, launch_mode=dg.DG_BATCH_PER_CLASS)
+        b.close()
         detail = {}
         el2, _, ss = W.run_sequential(cases, n, prec, detail=detail)
         for x in ss:
             x.close()
         out[name] = dict(steps=n, batched_dof_stage_per_s=float("%.4g" % (work / el)),
                          one_at_a_time_dof_stage_per_s=float("%.4g" % (work / el2)),
-                         speedup=round(el2 / el, 3), bounds=bounds(detail, el2, copy_gbs))
+                         speedup=round(el2 / el, 3),
+                         per_class_launches=dict(dof_stage_per_s=float("%.4g" % (work / elc)),
+                                                 speedup=round(el2 / elc, 3)),
+                         bounds=bounds(detail, el2, copy_gbs))
     out["kernel_choice"] = ("fixed per order as the autotuner picks at large sizes: lane items (one element per "
                             "lane) for p <= 2, matrix-core tiles for p >= 3")
+    out["launches"] = ("one persistent launch per RK stage (speedup); per_class_launches: one per stage and "
+                       "kernel class (Euler/advection x tiles/lane items), each at its own occupancy")
     return out
 
 

@@ -31,6 +31,7 @@ DG_IC_CONSTANT, DG_IC_GAUSSIAN, DG_IC_VORTEX, DG_IC_SUPERSONIC, DG_IC_POLY = 0, 
 DG_FLUX_LLF, DG_FLUX_CENTRAL = 0, 1
 DG_QUIRK_FLOAT_SQRT, DG_QUIRK_FLOAT_GEOM, DG_QUIRK_FLOAT_POW, DG_QUIRK_REFLECT_MIRROR = 1, 2, 4, 8
 DG_GMSH_QUIRKS, DG_GMSH_PERIODIC = 1, 2
+DG_BATCH_ONE_LAUNCH, DG_BATCH_PER_CLASS = 0, 1
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_IO", 3: "DG_ERR_PARSE", 4: "DG_ERR_HIP",
                 5: "DG_ERR_UNSUPPORTED", 6: "DG_ERR_NUMERIC", 7: "DG_ERR_BUDGET", 8: "DG_ERR_STATE"}
 STATUS_CODES = {v: k for k, v in STATUS_NAMES.items()}
@@ -114,6 +115,7 @@ def lib():
             "dg_solver_sync": (I, [VP]), "dg_solver_set_kernel": (I, [VP, I]), "dg_solver_get_kernel": (I, [VP, ip]),
             "dg_solver_set_max_groups": (I, [VP, I]), "dg_solver_launch_info": (I, [VP, ip]),
             "dg_batch_set_max_groups": (I, [VP, I]), "dg_batch_launch_info": (I, [VP, ip]),
+            "dg_batch_set_launch_mode": (I, [VP, I]),
             "dg_time_stage_kernels": (I, [VP, I, D, dp]), "dg_time_step_graph": (I, [VP, I, D, dp]),
             "dg_solver_device_bytes": (I, [VP, P(I64)]),
             "dg_bandwidth_probe": (I, [I64, I, dp]), "dg_bandwidth_probe_ex": (I, [I64, I, dp, I]),
@@ -510,8 +512,15 @@ class Batch:
         """Cap the persistent grid at n workgroups (0 = all resident)."""
         _check(lib().dg_batch_set_max_groups(self._h, n))
 
+    def set_launch_mode(self, mode):
+        """DG_BATCH_ONE_LAUNCH (default: one persistent launch per stage) or
+        DG_BATCH_PER_CLASS (one per kernel class and stage); from the next
+        run on."""
+        _check(lib().dg_batch_set_launch_mode(self._h, mode))
+
     def launch_info(self):
-        """dict(groups, tile_items, lane_items, longest_run) of the uploaded work list."""
+        """dict(groups, tile_items, lane_items, longest_run) of the uploaded
+        work list (groups: the largest launch)."""
         o = np.zeros(4, np.int32)
         _check(lib().dg_batch_launch_info(self._h, _ip(o)))
         return dict(groups=int(o[0]), tile_items=int(o[1]), lane_items=int(o[2]), longest_run=int(o[3]))

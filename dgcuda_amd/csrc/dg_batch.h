@@ -1,5 +1,6 @@
-// Many independent solves of mixed size and order advanced together: one
-// persistent launch per RK stage over a device-side work list.
+// Many independent solves of mixed size and order advanced together:
+// persistent launches per RK stage (one per kernel class, or one in all)
+// over device-side work lists.
 #pragma once
 #include "dg_solver.h"
 
@@ -17,6 +18,12 @@ class BatchEngine {
     // (per-solve dt and status) but run their own single-solve stage
     // kernels inside the captured step, after the batch launch.
     virtual void set_solo_threshold(int min_elems) = 0;
+    // true (default): a single persistent launch per stage carrying every
+    // body; false: one per stage for each kernel class present (Euler
+    // tiles, advection tiles, Euler lane items, advection lane items), each
+    // with the register budget of its own bodies.  Takes effect at the next
+    // set_slots.
+    virtual void set_one_launch(bool one) = 0;
     // enqueues one step (per-solve dt + 4 stages) for every resident solve
     virtual Status step(bool use_graph) = 0;
     // per slot: done flag, steps, t (waits for the engine's stream)
@@ -27,8 +34,9 @@ class BatchEngine {
     // caps the persistent grid at max_groups workgroups (a multiple of 8, at
     // least 8; 0 = every resident workgroup)
     virtual Status set_max_groups(int max_groups) = 0;
-    // out[0] workgroups launched, out[1] tile items, out[2] lane items,
-    // out[3] longest run of consecutive tiles of one solve a workgroup takes
+    // out[0] workgroups of the largest launch, out[1] tile items, out[2]
+    // lane items, out[3] longest run of consecutive tiles of one solve a
+    // workgroup takes
     virtual void launch_info(int* out) const = 0;
 };
 
@@ -48,6 +56,7 @@ class Batch {
     virtual void info(int* n_solves, int64_t* total_elems, int64_t* device_bytes) const = 0;
     virtual Status set_max_groups(int max_groups) = 0;
     virtual void launch_info(int* out) const = 0;
+    virtual void set_one_launch(bool one) = 0;
 };
 
 }  // namespace dgmi

@@ -1,7 +1,9 @@
 // Many independent solves of uneven size and mixed order advanced together:
-// one persistent launch per RK stage over a device-side work list of
-// (solve, tile) items, the step (per-solve dt kernel + 4 stage launches)
-// replayed from a hipGraph.
+// persistent launches per RK stage over device-side work lists of (solve,
+// tile) items, the step (per-solve dt kernel + the stage launches) replayed
+// from a hipGraph.  By default each kernel class present (Euler tiles,
+// advection tiles, Euler lane items, advection lane items) is one launch per
+// stage; the one-launch form runs every class in a single kernel per stage.
 //
 // Every solve keeps its own buffers, geometry and control block (its own
 // CFL dt, time and end time), so a solve's arithmetic does not depend on
@@ -9,14 +11,15 @@
 // any mix.  Solves of p >= 3 run as 64-element tiles through the pipelined
 // matrix-core body of the single-solve kernel (FP32 or FP64), solves of
 // p <= 2 as 256-element lane items of the one-element-per-lane body, the
-// single solve's kernel choice in both cases.  The work list is split into 8
-// contiguous chunks of equal estimated cost, one per XCD, and the
+// single solve's kernel choice in both cases.  Each work list is split into
+// 8 contiguous chunks of equal estimated cost, one per XCD, and the
 // workgroups of an XCD take the items of its chunk round-robin.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
 
 #include "dg_batch.h"
+#include "dg_worklist.h"
 #include "dg_mfma_body.h"
 #include "dg_valu_body.h"
 
@@ -149,16 +152,30 @@ __device__ __forceinline__ StageArgs<R> slot_args(const SlotDesc<R>* sd) {
     return a;
 }
 
-// Work-list layout in `parts` (XCD x = blockIdx % 8):
-//   parts[x] .. parts[x+1]          lane items of XCD x   (one element per lane)
-//   parts[9+x] .. parts[9+x+1]      tile items of XCD x   (matrix-core tiles)
-//   parts[18]                       resident slot count
-constexpr int BATCH_TILE_PART = 9;
-constexpr int BATCH_NSLOT_AT = 18;
-constexpr int BATCH_PARTS = 19;
+// Kernel classes.  BC_ALL is the one-launch form (the default): lane items,
+// then tile items, of every solve in a single kernel per stage, so light
+// (advection) and heavy (Euler) items share the CUs.  In the per-class form
+// each class is a persistent launch of its own per stage, compiled with the
+// bodies of its class only, so that its register allocation, and with it
+// the wavefronts a SIMD holds, is that of those bodies (FP64: Euler tiles 2
+// and advection tiles 4 wavefronts per SIMD, where the one-launch kernel
+// holds 1).
+enum : int { BC_LANE_ADV = 0, BC_LANE_EUL = 1, BC_TILE_ADV = 2, BC_TILE_EUL = 3, BC_NCLASS = 4, BC_ALL = 4 };
 
-// Lane items: 256 elements each, one per thread.
-template <typename R, int KIND>
+// Work lists in `parts` (XCD x = blockIdx % 8), 9 bounds per list:
+//   parts[9l + x] .. parts[9l + x + 1]   items of list l on XCD x
+//   parts[36]                            resident slot count
+// Per-class form: list l = class l.  One-launch form: list 0 = every lane
+// item, list 1 = every tile item.  A launch is given parts + 9l; workgroup
+// bi of XCD x takes items parts[9l + x] + bi, + G8, + 2 G8, ... (G8 =
+// workgroups per XCD).
+constexpr int BATCH_LISTS = 4;
+constexpr int BATCH_NSLOT_AT = 9 * BATCH_LISTS;
+constexpr int BATCH_PARTS = BATCH_NSLOT_AT + 1;
+
+// Lane items: 256 elements each, one per thread; ADV / EUL select the
+// bodies compiled in (advection N = 1, Euler N = 4).
+template <typename R, int KIND, bool ADV, bool EUL>
 __device__ __forceinline__ void batch_lane_items(const SlotDesc<R>* __restrict__ slots,
                                                  const int2* __restrict__ items, int it, int i1, int G8,
                                                  double* sh) {
@@ -171,10 +188,14 @@ __device__ __forceinline__ void batch_lane_items(const SlotDesc<R>* __restrict__
         const int N = uni(sd->N), p = uni(sd->p);
         const int e = uni(item.y) * BATCH_VALU_TILE + threadIdx.x;
         double lam = 0.0;
-        if (N == 4 && p == 2) lam = valu_process<4, 2, R>(a, e, KIND);
-        else if (N == 4 && p == 1) lam = valu_process<4, 1, R>(a, e, KIND);
-        else if (N == 1 && p == 2) lam = valu_process<1, 2, R>(a, e, KIND);
-        else if (N == 1 && p == 1) lam = valu_process<1, 1, R>(a, e, KIND);
+        if constexpr (EUL) {
+            if (N == 4 && p == 2) lam = valu_process<4, 2, R>(a, e, KIND);
+            else if (N == 4 && p == 1) lam = valu_process<4, 1, R>(a, e, KIND);
+        }
+        if constexpr (ADV) {
+            if (N == 1 && p == 2) lam = valu_process<1, 2, R>(a, e, KIND);
+            else if (N == 1 && p == 1) lam = valu_process<1, 1, R>(a, e, KIND);
+        }
         if constexpr (LAST) {
             const double m = block_max(lam, sh);
             if (threadIdx.x == 0) lam_commit(m, a.lam_bits);
@@ -185,7 +206,7 @@ __device__ __forceinline__ void batch_lane_items(const SlotDesc<R>* __restrict__
 
 // Tile items of XCD x: runs of consecutive tiles of one solve through the
 // pipelined matrix-core body.
-template <typename R, int KIND>
+template <typename R, int KIND, bool ADV, bool EUL>
 __device__ __forceinline__ void batch_tile_items(const SlotDesc<R>* __restrict__ slots,
                                                  const int2* __restrict__ items, int it, int i1, int G8,
                                                  typename MfmaT<R>::V* fd, R* ts, typename MfmaT<R>::V* tabA,
@@ -209,10 +230,14 @@ __device__ __forceinline__ void batch_tile_items(const SlotDesc<R>* __restrict__
         const int N = uni(sd->N), p = uni(sd->p);
         double lam = 0.0;
         int next = it + G8;
-        if (N == 4 && p == 4) next = batch_mfma_run<4, 4, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
-        else if (N == 4 && p == 3) next = batch_mfma_run<4, 3, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
-        else if (N == 1 && p == 4) next = batch_mfma_run<1, 4, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
-        else if (N == 1 && p == 3) next = batch_mfma_run<1, 3, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        if constexpr (EUL) {
+            if (N == 4 && p == 4) next = batch_mfma_run<4, 4, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+            else if (N == 4 && p == 3) next = batch_mfma_run<4, 3, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        }
+        if constexpr (ADV) {
+            if (N == 1 && p == 4) next = batch_mfma_run<1, 4, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+            else if (N == 1 && p == 3) next = batch_mfma_run<1, 3, KIND, R>(a, mx, slot, items, it, i1, G8, fd, ts, tabA, lam);
+        }
         it = next;
         if constexpr (LAST) {
             const double m = block_max(lam, sh);
@@ -222,67 +247,74 @@ __device__ __forceinline__ void batch_tile_items(const SlotDesc<R>* __restrict__
     }
 }
 
-// One persistent launch per stage.  Each workgroup first takes its share of
-// its XCD's lane items, then of its tile items; the two walks are separate
-// loops so the register allocation of one body does not burden the other.
-// Solves use lane items for p <= 2 and tiles for p >= 3 (the kernel choice
-// of a single solve, so results match it bit for bit).
-// Workgroups per CU the kernels' registers are held to.  FP32: 2 (3 spills
-// 76-220 B per lane and measured no faster).  FP64: 1, which lets the
-// compiler keep its spills in the accumulation registers (AGPRs) of the
-// single wavefront per SIMD; 2 spills about 1 KB per lane to memory.
-#ifndef DGMI_BATCH_F32_MINW
-#define DGMI_BATCH_F32_MINW 2
+// Workgroups per CU each kernel's registers are held to, by class (lane
+// advection, lane Euler, tile advection, tile Euler, one launch).  One
+// launch: FP32 2 (3 spills 76-220 B per lane and measured no faster); FP64
+// 1, which lets the compiler keep its spills in the accumulation registers
+// (AGPRs) of the single wavefront per SIMD (2 spill about 1 KB per lane to
+// memory).  The class kernels: the settings measured fastest on the ragged
+// 64-solve batch among 2-6 per class.
+#ifndef DGMI_BATCH_GROUPS_F32
+#define DGMI_BATCH_GROUPS_F32 4, 3, 4, 3, 2
 #endif
-#ifndef DGMI_BATCH_F64_MINW
-#define DGMI_BATCH_F64_MINW 1
+#ifndef DGMI_BATCH_GROUPS_F64
+#define DGMI_BATCH_GROUPS_F64 3, 1, 4, 2, 1
 #endif
-template <int KIND>
-__global__ void __launch_bounds__(256, DGMI_BATCH_F32_MINW) k_batch_stage_f32(const SlotDesc<float>* __restrict__ slots,
-                                                           const int2* __restrict__ items,
-                                                           const int* __restrict__ parts) {
-    extern __shared__ float4 lds4[];
-    __shared__ double sh[8];
-    floatx4* fd = reinterpret_cast<floatx4*>(lds4);
-    float* ts = tile_traces<floatx4, float>(fd);
-    __shared__ floatx4 tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
-    const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
-    batch_lane_items<float, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
-    batch_tile_items<float, KIND>(slots, items, parts[BATCH_TILE_PART + x] + bi, parts[BATCH_TILE_PART + x + 1], G8,
-                                  fd, ts, tabA, sh);
+template <typename R, int CLS>
+constexpr int batch_min_groups() {
+    constexpr int f32[5] = {DGMI_BATCH_GROUPS_F32}, f64[5] = {DGMI_BATCH_GROUPS_F64};
+    return sizeof(R) == 4 ? f32[CLS] : f64[CLS];
 }
 
-template <int KIND>
-__global__ void __launch_bounds__(256, DGMI_BATCH_F64_MINW) k_batch_stage_f64(const SlotDesc<double>* __restrict__ slots,
-                                                           const int2* __restrict__ items,
-                                                           const int* __restrict__ parts) {
+// One persistent launch per stage and class (or, BC_ALL, per stage).  A
+// workgroup takes its share of its XCD's items of the launch's list(s);
+// lane and tile items are separate loops so the register allocation of one
+// body does not burden the other.  Solves use lane items for p <= 2 and
+// tiles for p >= 3 (the kernel choice of a single solve, so results match
+// it bit for bit).
+template <typename R, int KIND, int CLS>
+__global__ void __launch_bounds__(256, (batch_min_groups<R, CLS>()))
+    k_batch_stage(const SlotDesc<R>* __restrict__ slots, const int2* __restrict__ items,
+                  const int* __restrict__ parts) {
+    using V = typename MfmaT<R>::V;
     extern __shared__ float4 lds4[];
     __shared__ double sh[8];
-    doublex4* fd = reinterpret_cast<doublex4*>(lds4);
-    double* ts = tile_traces<doublex4, double>(fd);
-    __shared__ doublex4 tabA[MFMA_A_SLOTS * 64];
     const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
-    batch_lane_items<double, KIND>(slots, items, parts[x] + bi, parts[x + 1], G8, sh);
-    batch_tile_items<double, KIND>(slots, items, parts[BATCH_TILE_PART + x] + bi, parts[BATCH_TILE_PART + x + 1],
-                                   G8, fd, ts, tabA, sh);
+    if constexpr (CLS == BC_LANE_ADV || CLS == BC_LANE_EUL || CLS == BC_ALL)
+        batch_lane_items<R, KIND, CLS != BC_LANE_EUL, CLS != BC_LANE_ADV>(slots, items, parts[x] + bi, parts[x + 1],
+                                                                          G8, sh);
+    if constexpr (CLS == BC_TILE_ADV || CLS == BC_TILE_EUL || CLS == BC_ALL) {
+        constexpr int T = CLS == BC_ALL ? 9 : 0;  // the one-launch form's tiles are list 1
+        V* fd = reinterpret_cast<V*>(lds4);
+        R* ts = tile_traces<V, R>(fd);
+        __shared__ V tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
+        batch_tile_items<R, KIND, CLS != BC_TILE_EUL, CLS != BC_TILE_ADV>(
+            slots, items, parts[T + x] + bi, parts[T + x + 1], G8, fd, ts, tabA, sh);
+    }
 }
 
 template <typename R, int KIND>
-struct BatchKernel;
-template <int KIND>
-struct BatchKernel<float, KIND> {
-    static const void* fn() { return reinterpret_cast<const void*>(&k_batch_stage_f32<KIND>); }
-    static void launch(dim3 g, size_t lds, hipStream_t st, const SlotDesc<float>* s, const int2* i, const int* x) {
-        hipLaunchKernelGGL((k_batch_stage_f32<KIND>), g, dim3(256), lds, st, s, i, x);
+const void* batch_stage_fn_k(int cls) {
+    switch (cls) {
+        case BC_LANE_ADV: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_LANE_ADV>);
+        case BC_LANE_EUL: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_LANE_EUL>);
+        case BC_TILE_ADV: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_TILE_ADV>);
+        case BC_TILE_EUL: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_TILE_EUL>);
+        default: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_ALL>);
     }
-};
-template <int KIND>
-struct BatchKernel<double, KIND> {
-    static const void* fn() { return reinterpret_cast<const void*>(&k_batch_stage_f64<KIND>); }
-    static void launch(dim3 g, size_t lds, hipStream_t st, const SlotDesc<double>* s, const int2* i, const int* x) {
-        hipLaunchKernelGGL((k_batch_stage_f64<KIND>), g, dim3(256), lds, st, s, i, x);
+}
+
+// the stage kernel of RK4 stage `kind` (ST_RK4_1..4) and class `cls`
+template <typename R>
+const void* batch_stage_fn(int kind, int cls) {
+    switch (kind) {
+        case ST_RK4_1: return batch_stage_fn_k<R, ST_RK4_1>(cls);
+        case ST_RK4_2: return batch_stage_fn_k<R, ST_RK4_2>(cls);
+        case ST_RK4_3: return batch_stage_fn_k<R, ST_RK4_3>(cls);
+        case ST_RK4_4: return batch_stage_fn_k<R, ST_RK4_4>(cls);
+        default: return nullptr;
     }
-};
+}
 
 template <typename R>
 __global__ void k_batch_dt(const SlotDesc<R>* __restrict__ slots, const int* __restrict__ nslots) {
@@ -308,26 +340,40 @@ __global__ void k_batch_status(const SlotDesc<R>* __restrict__ slots, int nslots
         if (_e != hipSuccess) return hip_status(_e, #expr);  \
     } while (0)
 
-// Device side of a batch: slot table, work list, XCD partition, the
-// persistent launch shape and the captured step.  The grid and LDS size are
-// fixed at creation (LDS for the largest tile halo), and the slot count
-// lives on the device, so the step graph stays valid while set_slots
-// changes which solves are resident, as long as the capacities hold.
+// Device side of a batch: slot table, work lists, XCD partition, the
+// persistent launches of a stage and the captured step.  A launch's grid is
+// the workgroups its kernel keeps resident (or the set_max_groups cap) and
+// its LDS fits the largest tile halo of its solves; the slot count lives on
+// the device.  The step graph stays valid while set_slots changes which
+// solves are resident, as long as the capacities and the launches (class,
+// grid, LDS) stay the same.
 template <typename R>
 class EngineT : public BatchEngine {
    public:
+    struct Launch {
+        int list = 0;     // its items: list `list` (and, BC_ALL, list 1 for the tiles)
+        int cls = BC_ALL;
+        int grid = 0;     // workgroups launched
+        int resident = 0; // workgroups of this kernel that fit on the device at once
+        size_t lds = 0;   // dynamic LDS per workgroup
+        bool operator==(const Launch& o) const {
+            return list == o.list && cls == o.cls && grid == o.grid && lds == o.lds;
+        }
+    };
     hipStream_t st = nullptr;
     SlotDesc<R>* d_slots = nullptr;
     int2* d_items = nullptr;
-    int* d_xcd = nullptr;   // BATCH_PARTS ints: lane and tile XCD chunk bounds, slot count
+    int* d_xcd = nullptr;   // BATCH_PARTS ints: XCD chunk bounds of the work lists, slot count
     double* d_status = nullptr;
     int cap_slots = 0, cap_items = 0;
-    int nslots = 0, nitems = 0, grid = 0, dt_grid = 0;
-    int grid_resident = 0;           // workgroups that fit on the device at once
+    int nslots = 0, nitems = 0, dt_grid = 0, cus = 0;
     int max_groups_req = 0;          // set_max_groups (0: no cap)
-    std::vector<int2> h_items;       // host copy of the work list and its XCD bounds
+    bool one_launch = true;          // set_one_launch: every class in one kernel per stage
+    std::vector<Launch> launches;    // the batch kernels of a stage, in launch order
+    std::vector<int2> raw_items[BATCH_LISTS];  // the work lists in solve order, and
+    std::vector<double> raw_cost[BATCH_LISTS]; // each item's estimated cost
+    std::vector<int2> h_items;       // host copy of the work lists and their XCD bounds
     std::vector<int> h_parts;
-    size_t lds = 0;
     hipGraphExec_t gexec = nullptr;
     int solo_min = 0;             // 0: every solve in the batch launch
     std::vector<Solver*> solo;    // solves launched with their own kernels
@@ -348,39 +394,38 @@ class EngineT : public BatchEngine {
         HIPST(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPST(hipMalloc((void**)&d_xcd, BATCH_PARTS * sizeof(int)));
         HIPST(hipMemsetAsync(d_xcd, 0, BATCH_PARTS * sizeof(int), st));
+        int dev = 0;
+        HIPST(hipGetDevice(&dev));
+        HIPST(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         if constexpr (sizeof(R) == 8) {  // above 64 KB: raise the kernels' dynamic LDS limit
-            int mxl = 0, dv = 0;
-            HIPST(hipGetDevice(&dv));
-            HIPST(hipDeviceGetAttribute(&mxl, hipDeviceAttributeMaxSharedMemoryPerBlock, dv));
-            for (const void* f : {BatchKernel<R, ST_RK4_1>::fn(), BatchKernel<R, ST_RK4_2>::fn(),
-                                  BatchKernel<R, ST_RK4_3>::fn(), BatchKernel<R, ST_RK4_4>::fn()})
-                HIPST(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, mxl - 16 * 1024));
+            int mxl = 0;
+            HIPST(hipDeviceGetAttribute(&mxl, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            for (int k : {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4})
+                for (int c = 0; c <= BC_ALL; c++)
+                    HIPST(hipFuncSetAttribute(batch_stage_fn<R>(k, c), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              mxl - 16 * 1024));
         }
-        Status s2 = shape_for(16);
-        if (!s2) return s2;
         Status s1 = reserve(max_slots, max_items);
         if (!s1) return s1;
         HIPST(hipStreamSynchronize(st));
         return Status::ok();
     }
 
-    // Dynamic LDS of the launch for the largest tile halo among the batched
-    // solves (FP64 with the largest halo allowed holds one workgroup per CU,
-    // with the usual halos two), and the persistent grid: the workgroups that
-    // fit at once, or the cap set by set_max_groups.  A change invalidates
-    // the captured step.
-    Status shape_for(int hcap) {
-        const size_t l = mfma_tile_lds_bytes(4, hcap, (int)sizeof(R));
-        int per_cu = 0, dev = 0, cus = 0;
-        HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BatchKernel<R, ST_RK4_2>::fn(), 256, l));
-        HIPST(hipGetDevice(&dev));
-        HIPST(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        const int res = std::max(8, per_cu * cus / 8 * 8);
-        const int g = max_groups_req == 0 ? res : std::min(res, max_groups_req / 8 * 8);
-        if (l != lds || g != grid) drop_graph();
-        lds = l;
-        grid_resident = res;
-        grid = g;
+    // LDS and grid of a launch: LDS for the largest tile halo among its
+    // solves (the Euler size in the one-launch form), the grid every
+    // workgroup that fits at once for all four stage kernels, or the cap set
+    // by set_max_groups.
+    Status shape(Launch& L, int hcap) {
+        const bool tiles = L.cls == BC_TILE_ADV || L.cls == BC_TILE_EUL || L.cls == BC_ALL;
+        L.lds = tiles ? mfma_tile_lds_bytes(L.cls == BC_TILE_ADV ? 1 : 4, hcap, (int)sizeof(R)) : 0;
+        int per_cu = 1 << 30;
+        for (int k : {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4}) {
+            int q = 0;
+            HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, batch_stage_fn<R>(k, L.cls), 256, L.lds));
+            per_cu = std::min(per_cu, q);
+        }
+        L.resident = std::max(8, per_cu * cus / 8 * 8);
+        L.grid = max_groups_req == 0 ? L.resident : std::min(L.resident, max_groups_req / 8 * 8);
         return Status::ok();
     }
 
@@ -416,13 +461,37 @@ class EngineT : public BatchEngine {
         solo_min = std::max(0, min_elems);
     }
 
+    void set_one_launch(bool one) override { one_launch = one; }
+
+    // Splits each list into 8 contiguous chunks of equal estimated cost, one
+    // per XCD, and uploads the lists with their bounds (stream-ordered;
+    // waits for the copies).
+    Status upload_lists() {
+        std::vector<int> xs(BATCH_PARTS, 0);
+        std::vector<int2> hi;
+        for (int l = 0; l < BATCH_LISTS; l++) {
+            xcd_chunks(raw_items[l], raw_cost[l], hi, &xs[9 * l]);
+        }
+        xs[BATCH_NSLOT_AT] = nslots;
+        const int ni = (int)hi.size();
+        Status s0 = reserve(std::max(nslots, 1), std::max(ni, 1));
+        if (!s0) return s0;
+        if (ni) HIPST(hipMemcpyAsync(d_items, hi.data(), ni * sizeof(int2), hipMemcpyHostToDevice, st));
+        HIPST(hipMemcpyAsync(d_xcd, xs.data(), BATCH_PARTS * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPST(hipStreamSynchronize(st));  // the host vectors go out of scope
+        nitems = ni;
+        h_items.swap(hi);
+        h_parts.swap(xs);
+        return Status::ok();
+    }
+
     Status set_slots(const std::vector<Solver*>& solvers) override {
         const int ns = (int)solvers.size();
         std::vector<Solver*> new_solo;
         std::vector<SlotDesc<R>> hs(ns);
-        std::vector<int2> lane, tile;
-        std::vector<double> lane_cost, tile_cost;
-        int hmax = 16;  // largest tile halo among the batched solves
+        std::vector<int2> list[BATCH_LISTS];
+        std::vector<double> cost[BATCH_LISTS];
+        int hmax[BATCH_LISTS] = {16, 16, 16, 16};  // largest tile halo per list
         for (int s = 0; s < ns; s++) {
             SlotInfo si;
             solvers[s]->slot_info(&si);
@@ -434,6 +503,8 @@ class EngineT : public BatchEngine {
                 return Status::err(DG_ERR_STATE, "batch slots use matrix-core tiles exactly when p >= 3");
             if (!solo_here && tiles && si.hcap > MFMA_MAX_HALO)
                 return Status::err(DG_ERR_UNSUPPORTED, "tile halo too large");
+            if (!solo_here && si.N != 1 && si.N != 4)
+                return Status::err(DG_ERR_STATE, "batch slots carry advection (1) or Euler (4) variables");
             SlotDesc<R>& d = hs[s];
             d.c = (R*)si.c;
             d.acc = (R*)si.acc;
@@ -458,49 +529,55 @@ class EngineT : public BatchEngine {
                 new_solo.push_back(solvers[s]);  // slot kept for dt and status; no work items
                 continue;
             }
-            if (tiles) hmax = std::max(hmax, si.hcap);
+            const int cls = (tiles ? BC_TILE_ADV : BC_LANE_ADV) + (si.N == 4 ? 1 : 0);
+            const int l = one_launch ? (tiles ? 1 : 0) : cls;
+            if (tiles) hmax[l] = std::max(hmax[l], si.hcap);
             const int te = tiles ? MFMA_TILE : BATCH_VALU_TILE;
             const int nt = (si.K + te - 1) / te;
             const double per_elem = (double)si.N * n_modes(si.p) * sizeof(R);
-            std::vector<int2>& list = tiles ? tile : lane;
-            std::vector<double>& cost = tiles ? tile_cost : lane_cost;
             for (int t = 0; t < nt; t++) {
-                list.push_back(int2{s, t});
-                cost.push_back(per_elem * std::min(te, si.K - t * te));
+                list[l].push_back(int2{s, t});
+                cost[l].push_back(per_elem * std::min(te, si.K - t * te));
             }
         }
-        Status sh = shape_for(hmax);
-        if (!sh) return sh;
-        // each list in 8 contiguous chunks of equal estimated cost, one per XCD
-        std::vector<int> xs(BATCH_PARTS, 0);
-        auto split = [&](const std::vector<double>& cost, int base, int off) {
-            const int n = (int)cost.size();
-            for (int k = 0; k <= 8; k++) xs[off + k] = base + n;
-            xs[off] = base;
-            const double total = std::accumulate(cost.begin(), cost.end(), 0.0);
-            double run = 0;
-            int x = 1;
-            for (int i = 0; i < n && x < 8; i++) {
-                run += cost[i];
-                while (x < 8 && run >= total * x / 8.0) xs[off + x++] = base + i + 1;
+        // the launches of a stage: one per non-empty class, or one in all
+        std::vector<Launch> nl;
+        if (one_launch) {
+            if (!list[0].empty() || !list[1].empty()) {
+                Launch L;
+                L.list = 0;
+                L.cls = BC_ALL;
+                Status sh = shape(L, hmax[1]);
+                if (!sh) return sh;
+                nl.push_back(L);
             }
-        };
-        split(lane_cost, 0, 0);
-        split(tile_cost, (int)lane.size(), BATCH_TILE_PART);
-        xs[BATCH_NSLOT_AT] = ns;
-        std::vector<int2> hi(lane);
-        hi.insert(hi.end(), tile.begin(), tile.end());
-        const int ni = (int)hi.size();
-        Status s0 = reserve(std::max(ns, 1), std::max(ni, 1));
+        } else {
+            // Euler tiles first: the longest items start earliest
+            for (int c : {BC_TILE_EUL, BC_TILE_ADV, BC_LANE_EUL, BC_LANE_ADV}) {
+                if (list[c].empty()) continue;
+                Launch L;
+                L.list = c;
+                L.cls = c;
+                Status sh = shape(L, hmax[c]);
+                if (!sh) return sh;
+                nl.push_back(L);
+            }
+        }
+        if (nl != launches) {
+            launches = nl;
+            drop_graph();
+        }
+        for (int l = 0; l < BATCH_LISTS; l++) {
+            raw_items[l].swap(list[l]);
+            raw_cost[l].swap(cost[l]);
+        }
+        nslots = ns;
+        Status s0 = reserve(std::max(ns, 1), 1);
         if (!s0) return s0;
         if (ns) HIPST(hipMemcpyAsync(d_slots, hs.data(), ns * sizeof(SlotDesc<R>), hipMemcpyHostToDevice, st));
-        if (ni) HIPST(hipMemcpyAsync(d_items, hi.data(), ni * sizeof(int2), hipMemcpyHostToDevice, st));
-        HIPST(hipMemcpyAsync(d_xcd, xs.data(), BATCH_PARTS * sizeof(int), hipMemcpyHostToDevice, st));
-        HIPST(hipStreamSynchronize(st));  // the host vectors go out of scope
-        nslots = ns;
-        nitems = ni;
-        h_items = hi;
-        h_parts = xs;
+        HIPST(hipStreamSynchronize(st));  // hs goes out of scope
+        Status s1 = upload_lists();
+        if (!s1) return s1;
         // The solo launches are part of the captured step, with the buffers of
         // the solves they were captured for; compare by id, since a new solve
         // can sit at the address of a deleted one.
@@ -515,15 +592,16 @@ class EngineT : public BatchEngine {
     }
 
     Status launch_stage(int kind) {
-        const dim3 g(grid);
-        switch (kind) {
-            case ST_RK4_1: BatchKernel<R, ST_RK4_1>::launch(g, lds, st, d_slots, d_items, d_xcd); break;
-            case ST_RK4_2: BatchKernel<R, ST_RK4_2>::launch(g, lds, st, d_slots, d_items, d_xcd); break;
-            case ST_RK4_3: BatchKernel<R, ST_RK4_3>::launch(g, lds, st, d_slots, d_items, d_xcd); break;
-            case ST_RK4_4: BatchKernel<R, ST_RK4_4>::launch(g, lds, st, d_slots, d_items, d_xcd); break;
-            default: return Status::err(DG_ERR_ARG, "unsupported batch stage");
+        for (const Launch& L : launches) {
+            const void* fn = batch_stage_fn<R>(kind, L.cls);
+            if (!fn) return Status::err(DG_ERR_ARG, "unsupported batch stage");
+            const SlotDesc<R>* slots = d_slots;
+            const int2* items = d_items;
+            const int* parts = d_xcd + 9 * L.list;
+            void* args[] = {(void*)&slots, (void*)&items, (void*)&parts};
+            HIPST(hipLaunchKernel(fn, dim3(L.grid), dim3(256), args, L.lds, st));
         }
-        return hip_status(hipGetLastError(), "batch stage launch");
+        return Status::ok();
     }
 
     Status step_direct() {
@@ -578,30 +656,45 @@ class EngineT : public BatchEngine {
     Status set_max_groups(int n) override {
         if (n < 0 || (n > 0 && n < 8)) return Status::err(DG_ERR_ARG, "max_groups must be 0 or at least 8");
         max_groups_req = n;
-        const int g = n == 0 ? grid_resident : std::min(grid_resident, n / 8 * 8);
-        if (g != grid) drop_graph();
-        grid = g;
+        bool changed = false;
+        for (Launch& L : launches) {
+            const int g = n == 0 ? L.resident : std::min(L.resident, n / 8 * 8);
+            changed = changed || g != L.grid;
+            L.grid = g;
+        }
+        if (changed) drop_graph();
         return Status::ok();
     }
 
-    // Workgroup bi of XCD x takes tile items parts[9 + x] + bi, + G8, ...;
-    // a run is a stretch of those that belong to one solve (batch_mfma_run).
+    // Workgroup bi of XCD x takes the positions parts[9l + x] + bi, + G8, ...
+    // of list l; a run is a stretch of those that belong to one solve
+    // (batch_mfma_run).
     void launch_info(int* o) const override {
-        o[0] = grid;
-        o[1] = o[2] = o[3] = 0;
+        o[0] = o[1] = o[2] = o[3] = 0;
         if (h_parts.size() < (size_t)BATCH_PARTS) return;
-        o[2] = h_parts[8] - h_parts[0];
-        o[1] = h_parts[BATCH_TILE_PART + 8] - h_parts[BATCH_TILE_PART];
-        const int G8 = grid / 8;
+        const auto count = [&](int l) { return (int)raw_items[l].size(); };
         int longest = 0;
-        for (int x = 0; x < 8; x++) {
-            const int i0 = h_parts[BATCH_TILE_PART + x], i1 = h_parts[BATCH_TILE_PART + x + 1];
-            for (int bi = 0; bi < G8; bi++) {
-                int run = 0, slot = -1;
-                for (int it = i0 + bi; it < i1; it += G8) {
-                    run = (h_items[it].x == slot) ? run + 1 : 1;
-                    slot = h_items[it].x;
-                    longest = std::max(longest, run);
+        for (const Launch& L : launches) {
+            o[0] = std::max(o[0], L.grid);
+            const bool all = L.cls == BC_ALL;
+            const bool tiles = L.cls == BC_TILE_ADV || L.cls == BC_TILE_EUL;
+            if (all) {
+                o[2] += count(0);
+                o[1] += count(1);
+            } else {
+                o[tiles ? 1 : 2] += count(L.list);
+            }
+            if (!all && !tiles) continue;
+            const int tl = all ? 1 : L.list, G8 = L.grid / 8;
+            for (int x = 0; x < 8; x++) {
+                const int i0 = h_parts[9 * tl + x], i1 = h_parts[9 * tl + x + 1];
+                for (int bi = 0; bi < G8; bi++) {
+                    int run = 0, slot = -1;
+                    for (int it = i0 + bi; it < i1; it += G8) {
+                        run = (h_items[it].x == slot) ? run + 1 : 1;
+                        slot = h_items[it].x;
+                        longest = std::max(longest, run);
+                    }
                 }
             }
         }
@@ -745,6 +838,10 @@ class BatchImpl : public Batch {
     }
 
     Status set_max_groups(int n) override { return eng->set_max_groups(n); }
+    void set_one_launch(bool one) override {
+        eng->set_one_launch(one);
+        dirty = true;  // the work lists are rebuilt at the next run
+    }
     // the work list is uploaded at the next run; describe that one
     void launch_info(int* out) const override { eng->launch_info(out); }
 

@@ -663,6 +663,15 @@ int dg_batch_set_max_groups(dg_batch* b, int max_groups) {
     GUARD_BEGIN return ret(b->b->set_max_groups(max_groups)); GUARD_END
 }
 
+int dg_batch_set_launch_mode(dg_batch* b, int mode) {
+    BATCH_CHECK(b);
+    if (mode != DG_BATCH_PER_CLASS && mode != DG_BATCH_ONE_LAUNCH)
+        return fail(DG_ERR_ARG, "mode must be DG_BATCH_PER_CLASS or DG_BATCH_ONE_LAUNCH");
+    GUARD_BEGIN b->b->set_one_launch(mode == DG_BATCH_ONE_LAUNCH);
+    return DG_OK;
+    GUARD_END
+}
+
 int dg_batch_launch_info(const dg_batch* b, int* out4) {
     BATCH_CHECK(b);
     if (!out4) return fail(DG_ERR_ARG, "null output");

@@ -130,9 +130,13 @@ def _ceiling(runs, elapsed, detail):
                   alone_s=float(sum(t for _, _, _, t in runs)))
 
 
-def run_batch(cases, steps, precision=DG_FP32, use_graph=True, warmup=2):
-    """All cases in one batch, `steps` RK4 steps each; returns (seconds, DOF*stage)."""
+def run_batch(cases, steps, precision=DG_FP32, use_graph=True, warmup=2, launch_mode=None):
+    """All cases in one batch, `steps` RK4 steps each; returns (seconds,
+    DOF*stage, batch).  launch_mode: DG_BATCH_ONE_LAUNCH (the default) or
+    DG_BATCH_PER_CLASS."""
     b = Batch(precision)
+    if launch_mode is not None:
+        b.set_launch_mode(launch_mode)
     for m, prob, p in cases:
         b.add(m, prob, p, 1e30)
     b.project_ic()

@@ -329,7 +329,7 @@ int dg_bandwidth_probe_ex(int64_t bytes, int iters, double* out, int nout);
  * reference). */
 int dg_selftest_mfma(double* max_err);
 
-/* ---------------- many solves, one persistent launch per RK stage ---------------- */
+/* ---------------- many solves, persistent launches per RK stage ---------------- */
 int dg_batch_create(int precision, dg_batch** out);
 void dg_batch_destroy(dg_batch* b);
 /* adds one solve (its own mesh, problem, order p = 1..4, end time); returns
@@ -341,7 +341,8 @@ int dg_batch_add(dg_batch* b, const dg_mesh* m, const dg_problem_desc* prob, int
 /* projects the initial conditions of every solve and resets their clocks */
 int dg_batch_project_ic(dg_batch* b);
 /* nsteps RK4 steps on every unfinished solve (each with its own CFL dt),
- * one persistent launch per stage, replayed from a hipGraph when use_graph */
+ * persistent launches per stage (dg_batch_set_launch_mode), replayed from a
+ * hipGraph when use_graph */
 int dg_batch_run_steps(dg_batch* b, int nsteps, double cfl, int use_graph);
 /* runs every solve to its end time; *steps = number of batch steps issued */
 int dg_batch_integrate(dg_batch* b, double cfl, int use_graph, int64_t* steps);
@@ -352,8 +353,21 @@ int dg_batch_info(const dg_batch* b, int* n_solves, int64_t* total_elems, int64_
  * a multiple of 8; 0 = every resident workgroup, the default).  Results do
  * not depend on it.  Test and tuning aid; no counterpart in the reference. */
 int dg_batch_set_max_groups(dg_batch* b, int max_groups);
+/* How a stage of the batch is launched; takes effect at the next run.
+ * DG_BATCH_ONE_LAUNCH (default): a single persistent launch per stage
+ * carrying every body, at the register budget of the largest; workgroups
+ * mix light and heavy items.  DG_BATCH_PER_CLASS: one persistent launch per
+ * kernel class present (Euler tiles, advection tiles, Euler lane items,
+ * advection lane items), each compiled with its own bodies only and so
+ * holding the wavefronts per SIMD those bodies allow (faster in FP64, whose
+ * one-launch kernel holds one wavefront per SIMD).  Results are the same
+ * bit for bit.  No counterpart in the reference (its solves run one at a
+ * time). */
+enum { DG_BATCH_ONE_LAUNCH = 0, DG_BATCH_PER_CLASS = 1 };
+int dg_batch_set_launch_mode(dg_batch* b, int mode);
 /* Launch shape of the last uploaded work list (uploaded by the first run
- * after dg_batch_add): out[0] workgroups, out[1] matrix-core tile items,
+ * after dg_batch_add): out[0] workgroups (of the largest launch), out[1]
+ * matrix-core tile items,
  * out[2] lane items, out[3] the longest run of consecutive tiles of one solve
  * that one workgroup takes through the pipelined tile body.  No counterpart
  * in the reference. */

@@ -1,6 +1,8 @@
 // Host stand-ins for the HIP names the device headers use, so that the lane
 // kernel body (dg_valu_body.h) compiles as plain C++ for a CPU bounds check
-// (tests/test_lane_kernel_host.py).  One "lane": no cross-lane operations.
+// (tests/test_lane_kernel_host.py), and the batch's work-list layout
+// (dg_worklist.h) for tests/test_worklist_host.py.  One "lane": no
+// cross-lane operations.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -19,6 +21,7 @@ typedef void* hipStream_t;
 enum { hipSuccess = 0 };
 struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
 struct Idx { unsigned x = 0, y = 0, z = 0; };
+struct int2 { int x, y; };
 static Idx threadIdx, blockIdx, blockDim, gridDim;
 template <typename T> inline hipError_t hipMemcpyToSymbol(T&, const void*, size_t) { return 0; }
 inline long long __double_as_longlong(double d) { long long x; memcpy(&x, &d, 8); return x; }

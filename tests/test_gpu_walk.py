@@ -187,3 +187,34 @@ def test_batch_runs_of_tiles_vs_single_and_oracle(prec):
     assert info["groups"] == 8 and info["longest_run"] >= 90
     for k in range(len(cases)):
         assert np.array_equal(bc.get_state(k), states[k]), k
+
+
+@pytest.mark.parametrize("prec", [dg.DG_FP32, dg.DG_FP64])
+def test_batch_per_class_launches_equal_one_launch(prec):
+    """The per-class form (one persistent launch per stage for each kernel
+    class: Euler tiles, advection tiles, Euler and advection lane items)
+    gives the one-launch form's results bit for bit, also with every class
+    kernel capped at 8 workgroups (long runs in each)."""
+    print("batch per-class prec=%d" % prec, flush=True)
+    cases = batch_cases()
+    got = {}
+    for mode, cap in ((dg.DG_BATCH_ONE_LAUNCH, 0), (dg.DG_BATCH_PER_CLASS, 0), (dg.DG_BATCH_PER_CLASS, 8)):
+        b = dg.Batch(prec)
+        b.set_launch_mode(mode)
+        for m, prob, p in cases:
+            b.add(m, prob, p, 1.0)
+        if cap:
+            b.set_max_groups(cap)
+        b.project_ic()
+        b.run_steps(3, 0.3)
+        info = b.launch_info()
+        print("  mode %d cap %d launch %s" % (mode, cap, info), flush=True)
+        assert info["tile_items"] > 0 and info["lane_items"] > 0
+        if cap:
+            assert info["groups"] == 8 and info["longest_run"] >= 90
+        got[(mode, cap)] = [b.get_state(k) for k in range(len(cases))]
+        b.close()
+    ref = got[(dg.DG_BATCH_ONE_LAUNCH, 0)]
+    for key, states in got.items():
+        for k in range(len(cases)):
+            assert np.array_equal(states[k], ref[k]), (key, k)
