@@ -28,6 +28,8 @@ class BatchEngine {
     virtual Status step(bool use_graph) = 0;
     // per slot: done flag, steps, t (waits for the engine's stream)
     virtual Status status(std::vector<double>& out) = 0;
+    // every slot's control block, in one copy (waits for the engine's stream)
+    virtual Status pull_ctrls(std::vector<Ctrl>& out) = 0;
     virtual Status sync() = 0;
     virtual hipStream_t stream() = 0;
     virtual int max_tile_halo() const = 0;

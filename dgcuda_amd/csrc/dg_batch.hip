@@ -322,6 +322,13 @@ __global__ void k_batch_dt(const SlotDesc<R>* __restrict__ slots, const int* __r
     if (s < *nslots) dt_update(slots[s].ctrl);
 }
 
+// every slot's control block into one array (one host copy)
+template <typename R>
+__global__ void k_batch_ctrls(const SlotDesc<R>* __restrict__ slots, int nslots, Ctrl* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslots) out[s] = *slots[s].ctrl;
+}
+
 // done flag, steps and t of every slot into one array (one host copy)
 template <typename R>
 __global__ void k_batch_status(const SlotDesc<R>* __restrict__ slots, int nslots, double* out) {
@@ -365,6 +372,7 @@ class EngineT : public BatchEngine {
     int2* d_items = nullptr;
     int* d_xcd = nullptr;   // BATCH_PARTS ints: XCD chunk bounds of the work lists, slot count
     double* d_status = nullptr;
+    Ctrl* d_ctrls = nullptr;
     int cap_slots = 0, cap_items = 0;
     int nslots = 0, nitems = 0, dt_grid = 0, cus = 0;
     int max_groups_req = 0;          // set_max_groups (0: no cap)
@@ -382,7 +390,7 @@ class EngineT : public BatchEngine {
     ~EngineT() override {
         if (st) (void)hipStreamSynchronize(st);
         if (gexec) hipGraphExecDestroy(gexec);
-        void* b[] = {d_slots, d_items, d_xcd, d_status};
+        void* b[] = {d_slots, d_items, d_xcd, d_status, d_ctrls};
         for (void* q : b)
             if (q) hipFree(q);
         if (st) hipStreamDestroy(st);
@@ -434,9 +442,14 @@ class EngineT : public BatchEngine {
         if (ns > cap_slots) {
             if (d_slots) hipFree(d_slots);
             if (d_status) hipFree(d_status);
+            if (d_ctrls) hipFree(d_ctrls);
+            d_slots = nullptr;
+            d_status = nullptr;
+            d_ctrls = nullptr;
             cap_slots = std::max(ns, 2 * cap_slots);
             HIPST(hipMalloc((void**)&d_slots, cap_slots * sizeof(SlotDesc<R>)));
             HIPST(hipMalloc((void**)&d_status, cap_slots * 3 * sizeof(double)));
+            HIPST(hipMalloc((void**)&d_ctrls, cap_slots * sizeof(Ctrl)));
             dt_grid = (cap_slots + 63) / 64;
             drop_graph();
         }
@@ -646,6 +659,16 @@ class EngineT : public BatchEngine {
         return Status::ok();
     }
 
+    Status pull_ctrls(std::vector<Ctrl>& out) override {
+        out.assign((size_t)nslots, Ctrl{});
+        if (nslots == 0) return sync();
+        hipLaunchKernelGGL((k_batch_ctrls<R>), dim3((nslots + 63) / 64), dim3(64), 0, st, d_slots, nslots, d_ctrls);
+        HIPST(hipGetLastError());
+        HIPST(hipMemcpyAsync(out.data(), d_ctrls, out.size() * sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+        HIPST(hipStreamSynchronize(st));
+        return Status::ok();
+    }
+
     Status sync() override {
         HIPST(hipStreamSynchronize(st));
         return Status::ok();
@@ -770,17 +793,21 @@ class BatchImpl : public Batch {
             if (!(st = eng->set_slots(solvers))) return st;
             dirty = false;
         }
+        // on the engine's stream, ahead of the steps: no wait per solve
         for (size_t s = 0; s < solvers.size(); s++)
-            if (!(st = solvers[s]->begin(t_end[s], has_end, cfl))) return st;
+            if (!(st = solvers[s]->begin_on(t_end[s], has_end, cfl, eng->stream()))) return st;
         return Status::ok();
     }
 
+    // every control block back in one copy (t, steps and the wave speed the
+    // next run starts from)
     Status finish_all() {
-        Status st = eng->sync();
+        std::vector<Ctrl> h;
+        Status st = eng->pull_ctrls(h);
         if (!st) return st;
-        for (Solver* s : solvers) {
-            if (!(st = s->refresh())) return st;
-            s->set_lambda_valid(true);
+        for (size_t s = 0; s < solvers.size(); s++) {
+            solvers[s]->set_host_ctrl(h[s]);
+            solvers[s]->set_lambda_valid(true);
         }
         return Status::ok();
     }

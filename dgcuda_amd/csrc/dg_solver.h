@@ -89,8 +89,13 @@ class Solver {
     // pushes the run parameters to the control block and computes the wave
     // speed of the resident state, then waits for the solver's stream
     virtual Status begin(double t_end, bool has_end, double cfl) = 0;
+    // the same, enqueued on stream `on` without waiting (the caller's later
+    // work on `on` sees the control block)
+    virtual Status begin_on(double t_end, bool has_end, double cfl, hipStream_t on) = 0;
     // reads the control block back (t, steps) after work issued elsewhere
     virtual Status refresh() = 0;
+    // takes a copy of the control block read back by the caller
+    virtual void set_host_ctrl(const Ctrl& c) = 0;
     // the last stage of a step issued elsewhere left the wave speed in ctrl
     virtual void set_lambda_valid(bool v) = 0;
     virtual hipStream_t get_stream() const = 0;

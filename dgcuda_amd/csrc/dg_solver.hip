@@ -648,8 +648,8 @@ class SolverT : public Solver {
         return Status::ok();
     }
 
-    Status push_ctrl() {
-        HIPST(hipMemcpyAsync(ctrl, &hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
+    Status push_ctrl(hipStream_t on = nullptr) {
+        HIPST(hipMemcpyAsync(ctrl, &hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, on ? on : stream));
         return Status::ok();
     }
 
@@ -696,9 +696,9 @@ class SolverT : public Solver {
         }
     }
 
-    Status ensure_lambda() {
+    Status ensure_lambda(hipStream_t on = nullptr) {
         if (lam_valid) return Status::ok();
-        HIPST(launch_lambda<R>(c, six, K, b0, phys, ctrl, stream));
+        HIPST(launch_lambda<R>(c, six, K, b0, phys, ctrl, on ? on : stream));
         lam_valid = true;
         return Status::ok();
     }
@@ -745,7 +745,7 @@ class SolverT : public Solver {
         return enqueue_stages(scheme);
     }
 
-    Status begin_run(double t_end, bool has_end, double cfl) {
+    Status begin_run(double t_end, bool has_end, double cfl, hipStream_t on = nullptr) {
         hctrl.t = t;
         hctrl.steps = steps;
         hctrl.t_end = t_end;
@@ -755,9 +755,9 @@ class SolverT : public Solver {
         hctrl.cfl = cfl;
         hctrl.min_r = min_r;
         hctrl.two_p1 = 2. * p + 1.;
-        Status st = push_ctrl();
+        Status st = push_ctrl(on);
         if (!st) return st;
-        return ensure_lambda();
+        return ensure_lambda(on);
     }
 
     Status project_ic() override {
@@ -1046,12 +1046,22 @@ class SolverT : public Solver {
         return Status::ok();
     }
 
+    Status begin_on(double t_end, bool has_end, double cfl, hipStream_t on) override {
+        return begin_run(t_end, has_end, cfl, on);
+    }
+
     Status refresh() override {
         Status st = pull_ctrl();
         if (!st) return st;
         t = hctrl.t;
         steps = hctrl.steps;
         return Status::ok();
+    }
+
+    void set_host_ctrl(const Ctrl& cc) override {
+        hctrl = cc;
+        t = hctrl.t;
+        steps = hctrl.steps;
     }
 
     void set_lambda_valid(bool v) override { lam_valid = v; }
