@@ -255,6 +255,8 @@ def main():
                     help="with --profile-steps: replay the captured step graph, as the timed run does")
     ap.add_argument("--profile-batch", default="", choices=["", "fp32", "fp64"],
                     help="with --profile-steps: run only the ragged batch (C4) in this precision")
+    ap.add_argument("--profile-per-class", action="store_true",
+                    help="with --profile-batch: one launch per stage and kernel class")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)" % args.gpus,
@@ -270,7 +272,8 @@ def main():
 
     if args.profile_steps and args.profile_batch:
         bprec = dg.DG_FP32 if args.profile_batch == "fp32" else dg.DG_FP64
-        el, work, b = W.run_batch(W.ragged_batch_c4(), args.profile_steps, bprec)
+        mode = dg.DG_BATCH_PER_CLASS if args.profile_per_class else dg.DG_BATCH_ONE_LAUNCH
+        el, work, b = W.run_batch(W.ragged_batch_c4(), args.profile_steps, bprec, launch_mode=mode)
         b.close()
         print("profiled %d batch steps in %.4f s" % (args.profile_steps, el), flush=True)
         return

@@ -18,12 +18,10 @@ class BatchEngine {
     // (per-solve dt and status) but run their own single-solve stage
     // kernels inside the captured step, after the batch launch.
     virtual void set_solo_threshold(int min_elems) = 0;
-    // true (default): a single persistent launch per stage carrying every
-    // body; false: one per stage for each kernel class present (Euler
-    // tiles, advection tiles, Euler lane items, advection lane items), each
-    // with the register budget of its own bodies.  Takes effect at the next
-    // set_slots.
-    virtual void set_one_launch(bool one) = 0;
+    // DG_BATCH_ONE_LAUNCH (default) or DG_BATCH_PER_CLASS (dgmi.h): which
+    // persistent launches a stage runs.
+    // Takes effect at the next set_slots.
+    virtual void set_launch_form(int form) = 0;
     // enqueues one step (per-solve dt + 4 stages) for every resident solve
     virtual Status step(bool use_graph) = 0;
     // per slot: done flag, steps, t (waits for the engine's stream)
@@ -58,7 +56,7 @@ class Batch {
     virtual void info(int* n_solves, int64_t* total_elems, int64_t* device_bytes) const = 0;
     virtual Status set_max_groups(int max_groups) = 0;
     virtual void launch_info(int* out) const = 0;
-    virtual void set_one_launch(bool one) = 0;
+    virtual void set_launch_form(int form) = 0;
 };
 
 }  // namespace dgmi

@@ -152,23 +152,33 @@ __device__ __forceinline__ StageArgs<R> slot_args(const SlotDesc<R>* sd) {
     return a;
 }
 
-// Kernel classes.  BC_ALL is the one-launch form (the default): lane items,
-// then tile items, of every solve in a single kernel per stage, so light
-// (advection) and heavy (Euler) items share the CUs.  In the per-class form
-// each class is a persistent launch of its own per stage, compiled with the
-// bodies of its class only, so that its register allocation, and with it
-// the wavefronts a SIMD holds, is that of those bodies (FP64: Euler tiles 2
-// and advection tiles 4 wavefronts per SIMD, where the one-launch kernel
-// holds 1).
-enum : int { BC_LANE_ADV = 0, BC_LANE_EUL = 1, BC_TILE_ADV = 2, BC_TILE_EUL = 3, BC_NCLASS = 4, BC_ALL = 4 };
+// Work-item classes, as bits of a launch's mask: the bodies a stage kernel
+// carries.  A batch runs each stage as one or more persistent launches whose
+// masks cover the classes present:
+//   one launch (the default): BM_ALL, so light (advection) and heavy (Euler)
+//     items share the CUs;
+//   per class: one launch per class, each with the register budget of its
+//     own bodies.
+// A kernel's register allocation, and with it the wavefronts a SIMD holds,
+// is that of the bodies it carries (FP64: Euler tiles alone 2 wavefronts per
+// SIMD, every body together 1).
+enum : int {
+    BM_LANE_ADV = 1,
+    BM_LANE_EUL = 2,
+    BM_TILE_ADV = 4,
+    BM_TILE_EUL = 8,
+    BM_LANES = BM_LANE_ADV | BM_LANE_EUL,
+    BM_TILES = BM_TILE_ADV | BM_TILE_EUL,
+    BM_ALL = BM_LANES | BM_TILES
+};
 
 // Work lists in `parts` (XCD x = blockIdx % 8), 9 bounds per list:
 //   parts[9l + x] .. parts[9l + x + 1]   items of list l on XCD x
 //   parts[36]                            resident slot count
-// Per-class form: list l = class l.  One-launch form: list 0 = every lane
-// item, list 1 = every tile item.  A launch is given parts + 9l; workgroup
-// bi of XCD x takes items parts[9l + x] + bi, + G8, + 2 G8, ... (G8 =
-// workgroups per XCD).
+// A launch is given parts + 9l: its lane items are list l, its tile items
+// list l + 1 if it carries lane bodies too, else list l.  Workgroup bi of
+// XCD x takes items parts[9l + x] + bi, + G8, + 2 G8, ... (G8 = workgroups
+// per XCD).
 constexpr int BATCH_LISTS = 4;
 constexpr int BATCH_NSLOT_AT = 9 * BATCH_LISTS;
 constexpr int BATCH_PARTS = BATCH_NSLOT_AT + 1;
@@ -247,71 +257,75 @@ __device__ __forceinline__ void batch_tile_items(const SlotDesc<R>* __restrict__
     }
 }
 
-// Workgroups per CU each kernel's registers are held to, by class (lane
-// advection, lane Euler, tile advection, tile Euler, one launch).  One
+// Workgroups per CU each kernel's registers are held to, by mask.  One
 // launch: FP32 2 (3 spills 76-220 B per lane and measured no faster); FP64
 // 1, which lets the compiler keep its spills in the accumulation registers
 // (AGPRs) of the single wavefront per SIMD (2 spill about 1 KB per lane to
-// memory).  The class kernels: the settings measured fastest on the ragged
-// 64-solve batch among 2-6 per class.
-#ifndef DGMI_BATCH_GROUPS_F32
-#define DGMI_BATCH_GROUPS_F32 4, 3, 4, 3, 2
-#endif
-#ifndef DGMI_BATCH_GROUPS_F64
-#define DGMI_BATCH_GROUPS_F64 3, 1, 4, 2, 1
-#endif
-template <typename R, int CLS>
+// memory).  The others: the settings measured fastest on the ragged
+// 64-solve batch among 2-6.
+template <typename R, int MASK>
 constexpr int batch_min_groups() {
-    constexpr int f32[5] = {DGMI_BATCH_GROUPS_F32}, f64[5] = {DGMI_BATCH_GROUPS_F64};
-    return sizeof(R) == 4 ? f32[CLS] : f64[CLS];
+    const bool f32 = sizeof(R) == 4;
+    switch (MASK) {
+        case BM_ALL: return f32 ? 2 : 1;
+        case BM_LANE_ADV: return f32 ? 4 : 3;
+        case BM_LANE_EUL: return f32 ? 3 : 1;
+        case BM_TILE_ADV: return 4;
+        default: return f32 ? 3 : 2;  // BM_TILE_EUL
+    }
 }
 
-// One persistent launch per stage and class (or, BC_ALL, per stage).  A
+// One persistent launch per stage for each mask of the batch's form.  A
 // workgroup takes its share of its XCD's items of the launch's list(s);
 // lane and tile items are separate loops so the register allocation of one
 // body does not burden the other.  Solves use lane items for p <= 2 and
 // tiles for p >= 3 (the kernel choice of a single solve, so results match
 // it bit for bit).
-template <typename R, int KIND, int CLS>
-__global__ void __launch_bounds__(256, (batch_min_groups<R, CLS>()))
+template <typename R, int KIND, int MASK>
+__global__ void __launch_bounds__(256, (batch_min_groups<R, MASK>()))
     k_batch_stage(const SlotDesc<R>* __restrict__ slots, const int2* __restrict__ items,
                   const int* __restrict__ parts) {
     using V = typename MfmaT<R>::V;
+    constexpr bool LANES = (MASK & BM_LANES) != 0, TILES = (MASK & BM_TILES) != 0;
     extern __shared__ float4 lds4[];
     __shared__ double sh[8];
     const int x = blockIdx.x & 7, G8 = gridDim.x >> 3, bi = blockIdx.x >> 3;
-    if constexpr (CLS == BC_LANE_ADV || CLS == BC_LANE_EUL || CLS == BC_ALL)
-        batch_lane_items<R, KIND, CLS != BC_LANE_EUL, CLS != BC_LANE_ADV>(slots, items, parts[x] + bi, parts[x + 1],
-                                                                          G8, sh);
-    if constexpr (CLS == BC_TILE_ADV || CLS == BC_TILE_EUL || CLS == BC_ALL) {
-        constexpr int T = CLS == BC_ALL ? 9 : 0;  // the one-launch form's tiles are list 1
+    if constexpr (LANES)
+        batch_lane_items<R, KIND, (MASK & BM_LANE_ADV) != 0, (MASK & BM_LANE_EUL) != 0>(slots, items, parts[x] + bi,
+                                                                                       parts[x + 1], G8, sh);
+    if constexpr (TILES) {
+        constexpr int T = LANES ? 9 : 0;  // the tile list follows the lane list
         V* fd = reinterpret_cast<V*>(lds4);
         R* ts = tile_traces<V, R>(fd);
         __shared__ V tabA[MFMA_A_SLOTS * 64];  // operand table of the current run's order
-        batch_tile_items<R, KIND, CLS != BC_TILE_EUL, CLS != BC_TILE_ADV>(
+        batch_tile_items<R, KIND, (MASK & BM_TILE_ADV) != 0, (MASK & BM_TILE_EUL) != 0>(
             slots, items, parts[T + x] + bi, parts[T + x + 1], G8, fd, ts, tabA, sh);
     }
 }
 
+// the masks a batch's forms launch
+constexpr int BATCH_MASKS[] = {BM_ALL, BM_LANE_ADV, BM_LANE_EUL, BM_TILE_ADV, BM_TILE_EUL};
+
 template <typename R, int KIND>
-const void* batch_stage_fn_k(int cls) {
-    switch (cls) {
-        case BC_LANE_ADV: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_LANE_ADV>);
-        case BC_LANE_EUL: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_LANE_EUL>);
-        case BC_TILE_ADV: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_TILE_ADV>);
-        case BC_TILE_EUL: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_TILE_EUL>);
-        default: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BC_ALL>);
+const void* batch_stage_fn_k(int mask) {
+    switch (mask) {
+        case BM_ALL: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BM_ALL>);
+        case BM_LANE_ADV: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BM_LANE_ADV>);
+        case BM_LANE_EUL: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BM_LANE_EUL>);
+        case BM_TILE_ADV: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BM_TILE_ADV>);
+        case BM_TILE_EUL: return reinterpret_cast<const void*>(&k_batch_stage<R, KIND, BM_TILE_EUL>);
+        default: return nullptr;
     }
 }
 
-// the stage kernel of RK4 stage `kind` (ST_RK4_1..4) and class `cls`
+// the stage kernel of RK4 stage `kind` (ST_RK4_1..4) and mask `mask`
 template <typename R>
-const void* batch_stage_fn(int kind, int cls) {
+const void* batch_stage_fn(int kind, int mask) {
     switch (kind) {
-        case ST_RK4_1: return batch_stage_fn_k<R, ST_RK4_1>(cls);
-        case ST_RK4_2: return batch_stage_fn_k<R, ST_RK4_2>(cls);
-        case ST_RK4_3: return batch_stage_fn_k<R, ST_RK4_3>(cls);
-        case ST_RK4_4: return batch_stage_fn_k<R, ST_RK4_4>(cls);
+        case ST_RK4_1: return batch_stage_fn_k<R, ST_RK4_1>(mask);
+        case ST_RK4_2: return batch_stage_fn_k<R, ST_RK4_2>(mask);
+        case ST_RK4_3: return batch_stage_fn_k<R, ST_RK4_3>(mask);
+        case ST_RK4_4: return batch_stage_fn_k<R, ST_RK4_4>(mask);
         default: return nullptr;
     }
 }
@@ -358,14 +372,15 @@ template <typename R>
 class EngineT : public BatchEngine {
    public:
     struct Launch {
-        int list = 0;     // its items: list `list` (and, BC_ALL, list 1 for the tiles)
-        int cls = BC_ALL;
+        int list = 0;     // its first work list (lanes, then tiles when it carries both)
+        int mask = BM_ALL;
         int grid = 0;     // workgroups launched
         int resident = 0; // workgroups of this kernel that fit on the device at once
         size_t lds = 0;   // dynamic LDS per workgroup
         bool operator==(const Launch& o) const {
-            return list == o.list && cls == o.cls && grid == o.grid && lds == o.lds;
+            return list == o.list && mask == o.mask && grid == o.grid && lds == o.lds;
         }
+        int tile_list() const { return list + ((mask & BM_LANES) ? 1 : 0); }
     };
     hipStream_t st = nullptr;
     SlotDesc<R>* d_slots = nullptr;
@@ -376,7 +391,7 @@ class EngineT : public BatchEngine {
     int cap_slots = 0, cap_items = 0;
     int nslots = 0, nitems = 0, dt_grid = 0, cus = 0;
     int max_groups_req = 0;          // set_max_groups (0: no cap)
-    bool one_launch = true;          // set_one_launch: every class in one kernel per stage
+    int form = DG_BATCH_ONE_LAUNCH;  // set_launch_form
     std::vector<Launch> launches;    // the batch kernels of a stage, in launch order
     std::vector<int2> raw_items[BATCH_LISTS];  // the work lists in solve order, and
     std::vector<double> raw_cost[BATCH_LISTS]; // each item's estimated cost
@@ -409,8 +424,8 @@ class EngineT : public BatchEngine {
             int mxl = 0;
             HIPST(hipDeviceGetAttribute(&mxl, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
             for (int k : {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4})
-                for (int c = 0; c <= BC_ALL; c++)
-                    HIPST(hipFuncSetAttribute(batch_stage_fn<R>(k, c), hipFuncAttributeMaxDynamicSharedMemorySize,
+                for (int m : BATCH_MASKS)
+                    HIPST(hipFuncSetAttribute(batch_stage_fn<R>(k, m), hipFuncAttributeMaxDynamicSharedMemorySize,
                                               mxl - 16 * 1024));
         }
         Status s1 = reserve(max_slots, max_items);
@@ -420,16 +435,16 @@ class EngineT : public BatchEngine {
     }
 
     // LDS and grid of a launch: LDS for the largest tile halo among its
-    // solves (the Euler size in the one-launch form), the grid every
+    // solves (at the Euler size when it carries Euler tiles), the grid every
     // workgroup that fits at once for all four stage kernels, or the cap set
     // by set_max_groups.
     Status shape(Launch& L, int hcap) {
-        const bool tiles = L.cls == BC_TILE_ADV || L.cls == BC_TILE_EUL || L.cls == BC_ALL;
-        L.lds = tiles ? mfma_tile_lds_bytes(L.cls == BC_TILE_ADV ? 1 : 4, hcap, (int)sizeof(R)) : 0;
+        const bool tiles = (L.mask & BM_TILES) != 0;
+        L.lds = tiles ? mfma_tile_lds_bytes((L.mask & BM_TILE_EUL) ? 4 : 1, hcap, (int)sizeof(R)) : 0;
         int per_cu = 1 << 30;
         for (int k : {ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4}) {
             int q = 0;
-            HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, batch_stage_fn<R>(k, L.cls), 256, L.lds));
+            HIPST(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, batch_stage_fn<R>(k, L.mask), 256, L.lds));
             per_cu = std::min(per_cu, q);
         }
         L.resident = std::max(8, per_cu * cus / 8 * 8);
@@ -474,7 +489,16 @@ class EngineT : public BatchEngine {
         solo_min = std::max(0, min_elems);
     }
 
-    void set_one_launch(bool one) override { one_launch = one; }
+    void set_launch_form(int f) override { form = f; }
+
+    // the masks of the form's launches, in launch order (Euler tiles first:
+    // the longest items start earliest)
+    std::vector<int> form_masks() const {
+        switch (form) {
+            case DG_BATCH_PER_CLASS: return {BM_TILE_EUL, BM_TILE_ADV, BM_LANE_EUL, BM_LANE_ADV};
+            default: return {BM_ALL};
+        }
+    }
 
     // Splits each list into 8 contiguous chunks of equal estimated cost, one
     // per XCD, and uploads the lists with their bounds (stream-ordered;
@@ -505,6 +529,27 @@ class EngineT : public BatchEngine {
         std::vector<int2> list[BATCH_LISTS];
         std::vector<double> cost[BATCH_LISTS];
         int hmax[BATCH_LISTS] = {16, 16, 16, 16};  // largest tile halo per list
+        // the form's launches and the work list of each class: a launch's
+        // lane list, then its tile list
+        std::vector<Launch> forms;
+        int list_of[BM_ALL + 1] = {};
+        int next = 0;
+        for (int m : form_masks()) {
+            Launch L;
+            L.mask = m;
+            L.list = next;
+            if (m & BM_LANES) {
+                for (int bit : {BM_LANE_ADV, BM_LANE_EUL})
+                    if (m & bit) list_of[bit] = next;
+                next++;
+            }
+            if (m & BM_TILES) {
+                for (int bit : {BM_TILE_ADV, BM_TILE_EUL})
+                    if (m & bit) list_of[bit] = next;
+                next++;
+            }
+            forms.push_back(L);
+        }
         for (int s = 0; s < ns; s++) {
             SlotInfo si;
             solvers[s]->slot_info(&si);
@@ -542,8 +587,8 @@ class EngineT : public BatchEngine {
                 new_solo.push_back(solvers[s]);  // slot kept for dt and status; no work items
                 continue;
             }
-            const int cls = (tiles ? BC_TILE_ADV : BC_LANE_ADV) + (si.N == 4 ? 1 : 0);
-            const int l = one_launch ? (tiles ? 1 : 0) : cls;
+            const int bit = tiles ? (si.N == 4 ? BM_TILE_EUL : BM_TILE_ADV) : (si.N == 4 ? BM_LANE_EUL : BM_LANE_ADV);
+            const int l = list_of[bit];
             if (tiles) hmax[l] = std::max(hmax[l], si.hcap);
             const int te = tiles ? MFMA_TILE : BATCH_VALU_TILE;
             const int nt = (si.K + te - 1) / te;
@@ -553,28 +598,15 @@ class EngineT : public BatchEngine {
                 cost[l].push_back(per_elem * std::min(te, si.K - t * te));
             }
         }
-        // the launches of a stage: one per non-empty class, or one in all
+        // the launches of a stage: the form's masks that have items
         std::vector<Launch> nl;
-        if (one_launch) {
-            if (!list[0].empty() || !list[1].empty()) {
-                Launch L;
-                L.list = 0;
-                L.cls = BC_ALL;
-                Status sh = shape(L, hmax[1]);
-                if (!sh) return sh;
-                nl.push_back(L);
-            }
-        } else {
-            // Euler tiles first: the longest items start earliest
-            for (int c : {BC_TILE_EUL, BC_TILE_ADV, BC_LANE_EUL, BC_LANE_ADV}) {
-                if (list[c].empty()) continue;
-                Launch L;
-                L.list = c;
-                L.cls = c;
-                Status sh = shape(L, hmax[c]);
-                if (!sh) return sh;
-                nl.push_back(L);
-            }
+        for (const Launch& L0 : forms) {
+            const bool lanes = (L0.mask & BM_LANES) != 0, tiles = (L0.mask & BM_TILES) != 0;
+            if ((!lanes || list[L0.list].empty()) && (!tiles || list[L0.tile_list()].empty())) continue;
+            Launch L = L0;
+            Status sh = shape(L, tiles ? hmax[L.tile_list()] : 16);
+            if (!sh) return sh;
+            nl.push_back(L);
         }
         if (nl != launches) {
             launches = nl;
@@ -606,7 +638,7 @@ class EngineT : public BatchEngine {
 
     Status launch_stage(int kind) {
         for (const Launch& L : launches) {
-            const void* fn = batch_stage_fn<R>(kind, L.cls);
+            const void* fn = batch_stage_fn<R>(kind, L.mask);
             if (!fn) return Status::err(DG_ERR_ARG, "unsupported batch stage");
             const SlotDesc<R>* slots = d_slots;
             const int2* items = d_items;
@@ -699,16 +731,10 @@ class EngineT : public BatchEngine {
         int longest = 0;
         for (const Launch& L : launches) {
             o[0] = std::max(o[0], L.grid);
-            const bool all = L.cls == BC_ALL;
-            const bool tiles = L.cls == BC_TILE_ADV || L.cls == BC_TILE_EUL;
-            if (all) {
-                o[2] += count(0);
-                o[1] += count(1);
-            } else {
-                o[tiles ? 1 : 2] += count(L.list);
-            }
-            if (!all && !tiles) continue;
-            const int tl = all ? 1 : L.list, G8 = L.grid / 8;
+            if (L.mask & BM_LANES) o[2] += count(L.list);
+            if (!(L.mask & BM_TILES)) continue;
+            const int tl = L.tile_list(), G8 = L.grid / 8;
+            o[1] += count(tl);
             for (int x = 0; x < 8; x++) {
                 const int i0 = h_parts[9 * tl + x], i1 = h_parts[9 * tl + x + 1];
                 for (int bi = 0; bi < G8; bi++) {
@@ -865,8 +891,8 @@ class BatchImpl : public Batch {
     }
 
     Status set_max_groups(int n) override { return eng->set_max_groups(n); }
-    void set_one_launch(bool one) override {
-        eng->set_one_launch(one);
+    void set_launch_form(int form) override {
+        eng->set_launch_form(form);
         dirty = true;  // the work lists are rebuilt at the next run
     }
     // the work list is uploaded at the next run; describe that one

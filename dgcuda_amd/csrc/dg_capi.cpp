@@ -665,9 +665,9 @@ int dg_batch_set_max_groups(dg_batch* b, int max_groups) {
 
 int dg_batch_set_launch_mode(dg_batch* b, int mode) {
     BATCH_CHECK(b);
-    if (mode != DG_BATCH_PER_CLASS && mode != DG_BATCH_ONE_LAUNCH)
-        return fail(DG_ERR_ARG, "mode must be DG_BATCH_PER_CLASS or DG_BATCH_ONE_LAUNCH");
-    GUARD_BEGIN b->b->set_one_launch(mode == DG_BATCH_ONE_LAUNCH);
+    if (mode != DG_BATCH_ONE_LAUNCH && mode != DG_BATCH_PER_CLASS)
+        return fail(DG_ERR_ARG, "mode must be DG_BATCH_ONE_LAUNCH or DG_BATCH_PER_CLASS");
+    GUARD_BEGIN b->b->set_launch_form(mode);
     return DG_OK;
     GUARD_END
 }
