@@ -160,15 +160,71 @@ enum { BC_REFLECTING = 1, BC_OUTFLOW = 2, BC_INFLOW = 3 };
 #endif
 constexpr int VALU_BOUNDARY_MAX_P = DGMI_VALU_BOUNDARY_MAX_P;
 
+// k = dt M^-1 R(u) of the stage's input u; c = the step's base state u_n
 enum StageKind {
-    ST_RHS = 0,   // out = dt * M^-1 R(u)
-    ST_RK4_1 = 1, // acc = k/6;      out = c + k/2
-    ST_RK4_2 = 2, // acc += k/3;     out = c + k/2
-    ST_RK4_3 = 3, // acc += k/3;     out = c + k
-    ST_RK4_4 = 4, // c = c + (acc + k/6); wave-speed partials
+    ST_RHS = 0,   // out = k
+    ST_RK4_1 = 1, // u = c:  out = u2 = c + k/2
+    ST_RK4_2 = 2, // u = u2: out = u3 = c + k/2; acc = W (rk_update)
+    ST_RK4_3 = 3, // u = u3: out = u4 = c + k
+    ST_RK4_4 = 4, // u = u4: c = acc + u4/3 + k/6; wave-speed partials
     ST_RK2_1 = 5, // out = c + k/2
     ST_RK2_2 = 6, // c = c + k; wave-speed partials
 };
+
+// Classical RK4 (time_integrator.cu:221) on every stage kernel, written
+// with the stage states instead of the slopes.  With u2 = u_n + dt k1/2,
+// u3 = u_n + dt k2/2 and u4 = u_n + dt k3, the slopes are dt k1 = 2(u2 - u_n),
+// dt k3 = u4 - u_n, so
+//   u_n+1 = u_n + dt (k1 + 2 k2 + 2 k3 + k4)/6 = W + u4/3 + dt k4/6,
+//   W = (2 u_n + (u2 - u_n) + dt k2)/3 = 2 u_n/3 + dt k1/6 + dt k2/3.
+// The stage that makes u3 also has u2 (its input) and u_n, and writes W to
+// the accumulator; the last stage reads only its input u4 and W.  A step
+// moves 12 state arrays (reads 1, 2, 2, 2; writes 1, 2, 1, 1) instead of 16
+// with a running sum of the slopes.  The same RK4 polynomial; the sums are
+// grouped differently from the reference's u_n + (k1/6 + k2/3 + k3/3 + k4/6),
+// with the small terms summed first and one rounding per state-sized sum.
+// Divisions by 3 of state-sized values are rounded to nearest (div3): a
+// product with the rounded 1/3 would bias every step's state by 3e-8 in
+// FP32, which shows as drift of the conserved totals.
+// Per stage the lane holds two operands of the update, `c` and `aa`:
+//   stage 1: c = input u_n                 -> u2
+//   stage 2: c = u_n (read), aa = input u2  -> u3, and W to the accumulator
+//   stage 3: c = u_n (read)                 -> u4
+//   stage 4: c = input u4, aa = W (read)    -> u_n+1
+template <int KIND>
+struct RkNeeds {
+    static constexpr bool C_FROM_INPUT = (KIND == ST_RK4_1 || KIND == ST_RK4_4 || KIND == ST_RK2_1);
+    static constexpr bool ACC_FROM_INPUT = (KIND == ST_RK4_2);
+    static constexpr bool LOAD_C = (KIND == ST_RK4_2 || KIND == ST_RK4_3 || KIND == ST_RK2_2);
+    static constexpr bool LOAD_ACC = (KIND == ST_RK4_4);
+};
+
+// x / 3: the product with the rounded 1/3, corrected once with the exact
+// residual x - 3q (one fused multiply-add), which rounds to nearest
+template <typename R>
+__device__ __forceinline__ R div3(R x) {
+    const R r3 = R(1) / R(3);
+    const R q = x * r3;
+    return fma(fma(R(-3), q, x), r3, q);
+}
+
+template <int KIND, typename R>
+__device__ __forceinline__ R rk_update(R c, R aa, R k, R* acc_out) {
+    if constexpr (KIND == ST_RHS) {
+        return k;
+    } else if constexpr (KIND == ST_RK4_1 || KIND == ST_RK2_1) {
+        return c + R(0.5) * k;
+    } else if constexpr (KIND == ST_RK4_2) {
+        *acc_out = div3(fma(R(2), c, (aa - c) + k));
+        return c + R(0.5) * k;
+    } else if constexpr (KIND == ST_RK4_3) {
+        return c + k;
+    } else if constexpr (KIND == ST_RK4_4) {
+        return fma(k, R(1) / R(6), aa + div3(c));
+    } else {
+        return c + k;
+    }
+}
 
 // Placement of entry (variable v, mode m, element e) of a state array in HBM.
 //  * Mode-major (the lane-per-element kernels): (v*NP + m)*Kp + e, so the

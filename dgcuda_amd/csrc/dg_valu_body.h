@@ -11,7 +11,8 @@
 //                      elements of a side obtain bit-identical fluxes
 //                      (race-free, no atomics, no face buffers)
 //   gather + M^-1      (time_integrator.cu:130-188, M = J*I)
-//   RK stage update    (time_integrator.cu:194-218, 445-464)
+//   RK stage update    (time_integrator.cu:194-218, 445-464; RK4 in stage
+//                      states, rk_update)
 //   CFL wave speed     (euler.cu:252-278) of the new state on the last stage.
 // Euler up to p = 3 uses the normal-flux forms of the matrix-core kernels
 // (fewer operations and registers); otherwise the reference's operation
@@ -187,14 +188,18 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
                 for (int m = 0; m < NP; m++) gp(a.out)[(size_t)(v * NP + m) * Kp + e] = cj * tot[v][m];
     } else {
-        // The RK operands of each variable are loaded before its first
-        // store: the output can be the RK base itself (the last stage
-        // updates it in place), so a load after a store could not be moved
-        // above it and each would wait for its own round trip to memory.
-        // One round trip per variable instead of one per mode (grouping all
-        // variables made the batched kernel spill).
-        const bool c_in = (kind == ST_RK4_1 || kind == ST_RK2_1);
-        const bool acc_in = (kind == ST_RK4_2 || kind == ST_RK4_3 || kind == ST_RK4_4);
+        // RK4 in stage states (rk_update): per stage the update's operands
+        // are c (the input u, or u_n read) and aa (the input u, or W read).
+        // They are loaded per variable before its first store: the output
+        // can be the base state itself (the last stage writes u_n+1 over
+        // u_n), so a load after a store could not be moved above it and
+        // each would wait for its own round trip to memory.  One round trip
+        // per variable instead of one per mode (grouping all variables made
+        // the batched kernel spill).
+        const bool c_in = (kind == ST_RK4_1 || kind == ST_RK4_4 || kind == ST_RK2_1);
+        const bool aa_in = (kind == ST_RK4_2);
+        const bool load_c = (kind == ST_RK4_2 || kind == ST_RK4_3 || kind == ST_RK2_2);
+        const bool load_acc = (kind == ST_RK4_4);
         R c0[N];
 #pragma unroll
         for (int v = 0; v < N; v++) {
@@ -202,28 +207,23 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
             for (int m = 0; m < NP; m++) {
                 const size_t ix = (size_t)(v * NP + m) * Kp + ee;
-                cv[m] = c_in ? u[v][m] : gp(a.c)[ix];
-                av[m] = acc_in ? gp(a.acc)[ix] : R(0);
+                cv[m] = c_in ? u[v][m] : (load_c ? gp(a.c)[ix] : R(0));
+                av[m] = aa_in ? u[v][m] : (load_acc ? gp(a.acc)[ix] : R(0));
             }
 #pragma unroll
             for (int m = 0; m < NP; m++) {
                 const size_t ix = (size_t)(v * NP + m) * Kp + ee;
                 const R k = cj * tot[v][m];
-                const R c = cv[m];
-                R o;
-                if (kind == ST_RK4_1) {
-                    if (valid) gp(a.acc)[ix] = k / R(6);
-                    o = c + R(0.5) * k;
-                } else if (kind == ST_RK4_2 || kind == ST_RK4_3) {
-                    if (valid) gp(a.acc)[ix] = av[m] + k / R(3);
-                    o = (kind == ST_RK4_2) ? c + R(0.5) * k : c + R(1.0) * k;
-                } else if (kind == ST_RK4_4) {
-                    o = c + (av[m] + k / R(6));
-                } else if (kind == ST_RK2_1) {
-                    o = c + R(0.5) * k;
-                } else {
-                    o = c + k;
+                R o, w = R(0);
+                switch (kind) {
+                    case ST_RK4_1: o = rk_update<ST_RK4_1, R>(cv[m], av[m], k, &w); break;
+                    case ST_RK4_2: o = rk_update<ST_RK4_2, R>(cv[m], av[m], k, &w); break;
+                    case ST_RK4_3: o = rk_update<ST_RK4_3, R>(cv[m], av[m], k, &w); break;
+                    case ST_RK4_4: o = rk_update<ST_RK4_4, R>(cv[m], av[m], k, &w); break;
+                    case ST_RK2_1: o = rk_update<ST_RK2_1, R>(cv[m], av[m], k, &w); break;
+                    default: o = rk_update<ST_RK2_2, R>(cv[m], av[m], k, &w); break;
                 }
+                if (valid && kind == ST_RK4_2) gp(a.acc)[ix] = w;
                 if (valid) gp(a.out)[ix] = o;
                 if (m == 0) c0[v] = o;
             }

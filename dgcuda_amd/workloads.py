@@ -91,19 +91,18 @@ def dof_stage_per_step(mesh, prob, p):
 def kernel_bytes_per_elem(N, n_p, s, kernel):
     """Bytes the kernels of this build must move per element and stage
     (each input read once, each output written once; halo re-reads not
-    counted).  Matrix-core path (kernel 2/3), RK4 in stage states: state
-    passes 3.0*N*n_p*s (stage 1 reads u_n and writes u2; stage 2 reads u2
-    and u_n and writes u3 and W = (2 u_n + (u2 - u_n) + dt k2)/3; stage 3
-    reads u3 and u_n and writes u4; stage 4 reads u4 and W and writes
-    u_n+1: 2, 4, 3, 3 arrays), plus the packed geometry: per face one
-    record of 4 values (nx, ny, len, neighbour word), per element a map of
-    4 values (xr, yr, xs, ys) and J, 17*s bytes (68 in FP32).  Lane-per-element
-    path (kernel 1): the reference's grouping, where stage 4 also reads u_n
-    (4.25 passes), geometry 14*s (xr, yr, xs, ys, J; per face nx, ny, len)
-    and 3 32-bit neighbour words."""
+    counted).  Every kernel runs RK4 in stage states: state passes
+    3.0*N*n_p*s (stage 1 reads u_n and writes u2; stage 2 reads u2 and u_n
+    and writes u3 and W = (2 u_n + (u2 - u_n) + dt k2)/3; stage 3 reads u3
+    and u_n and writes u4; stage 4 reads u4 and W and writes u_n+1: 2, 4,
+    3, 3 arrays).  Matrix-core path (kernel 2/3): the packed geometry, per
+    face one record of 4 values (nx, ny, len, neighbour word), per element a
+    map of 4 values (xr, yr, xs, ys) and J, 17*s bytes (68 in FP32).
+    Lane-per-element path (kernel 1): geometry 14*s (xr, yr, xs, ys, J; per
+    face nx, ny, len) and 3 32-bit neighbour words."""
     if kernel >= 2:
         return 3.0 * N * n_p * s + (3 * 4 + 4 + 1) * s
-    return 4.25 * N * n_p * s + 14 * s + 3 * 4
+    return 3.0 * N * n_p * s + 14 * s + 3 * 4
 
 
 def bytes_per_step(mesh, prob, p, precision, kernel):

@@ -79,5 +79,5 @@ def test_kernel_byte_model():
     # Euler p = 4 FP32 on the matrix-core path: 12 state arrays per step
     # (3 per stage) of 60 values and 68 B of packed geometry per element
     assert W.kernel_bytes_per_elem(4, 15, 4, 2) == 3 * 60 * 4 + 68
-    # lane path: 4.25 state passes, 14 geometry values, 3 neighbour words
-    assert W.kernel_bytes_per_elem(4, 3, 4, 1) == 4.25 * 12 * 4 + 14 * 4 + 12
+    # lane path: the same 3 state passes, 14 geometry values, 3 neighbour words
+    assert W.kernel_bytes_per_elem(4, 3, 4, 1) == 3 * 12 * 4 + 14 * 4 + 12
