@@ -239,7 +239,7 @@ template <typename R = float>
 struct TileGeo {
     using V = typename MfmaT<R>::V;
     V f;
-    R xr, yr, xs, ys, J;
+    R xr, yr, xs, ys, ij;  // ij: the inverse mass 1/J (M = J*I), precomputed in FP64
     __device__ __forceinline__ void load(const StageArgs<R>& a, const MfmaArgs& mx, int tile, int el) {
         f = ((const DG_GLOBAL V*)mx.face4)[(unsigned)(tile * (MFMA_TILE * 3) + min((int)threadIdx.x, MFMA_TILE * 3 - 1))];
         const unsigned ue = (unsigned)(tile * MFMA_TILE + el);
@@ -248,7 +248,7 @@ struct TileGeo {
         yr = m[1];
         xs = m[2];
         ys = m[3];
-        J = (gp(a.geo_e) + 4 * (size_t)a.Kp)[ue];
+        ij = (gp(a.geo_e) + 4 * (size_t)a.Kp)[ue];
     }
 };
 
@@ -256,8 +256,6 @@ struct TileGeo {
 // its value)
 __device__ __forceinline__ int face_word(float x) { return __float_as_int(x); }
 __device__ __forceinline__ int face_word(double x) { return (int)x; }
-__device__ __forceinline__ float inv_jacobian(float J) { return __builtin_amdgcn_rcpf(J); }
-__device__ __forceinline__ double inv_jacobian(double J) { return 1.0 / J; }
 
 // LLF flux of one point of one side: face record `r`, the element's trace
 // at the point `own` and the neighbour's `nbt`, in the orientation of the
@@ -542,7 +540,7 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
 
     // geometry of the next tile and halo ids of the one after, requested
     // before this tile's stores
-    const R cj = inv_jacobian(gc.J) * dt;
+    const R cj = gc.ij * dt;
     __builtin_amdgcn_s_setprio(2);
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (sizeof(R) == 8) {

@@ -395,7 +395,8 @@ class SolverT : public Solver {
         HIPST(hipMemsetAsync(acc, 0, M * sizeof(R), stream));
         HIPST(hipMemsetAsync(ua, 0, M * sizeof(R), stream));
         HIPST(hipMemsetAsync(ub, 0, M * sizeof(R), stream));
-        // geometry, padded entries replicate the last element
+        // geometry, padded entries replicate the last element; the 5th
+        // element value is the inverse mass 1/J (M = J*I), computed in FP64
         std::vector<R> he(5 * (size_t)Kp), hf(9 * (size_t)Kp);
         std::vector<int> hn(3 * (size_t)Kp);
         std::vector<double> hv(6 * (size_t)Kp), hj(Kp);
@@ -406,7 +407,7 @@ class SolverT : public Solver {
             he[Kp + i] = gr(g.yr[e]);
             he[2 * (size_t)Kp + i] = gr(g.xs[e]);
             he[3 * (size_t)Kp + i] = gr(g.ys[e]);
-            he[4 * (size_t)Kp + i] = gr(g.J[e]);
+            he[4 * (size_t)Kp + i] = gr(1.0 / g.J[e]);
             hj[i] = g.J[e];
             for (int f = 0; f < 3; f++) {
                 hf[(3 * f + 0) * (size_t)Kp + i] = gr(g.fnx[f][e]);

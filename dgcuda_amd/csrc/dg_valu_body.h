@@ -177,9 +177,8 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
             }
     }
 
-    // ---- inverse mass (J*I) and the RK stage update ----
-    const R J = gp(a.geo_e)[4 * Kp + ee];
-    const R cj = R(1) / J * dt;
+    // ---- inverse mass (M = J*I; 1/J precomputed in FP64) and the RK stage update ----
+    const R cj = gp(a.geo_e)[4 * Kp + ee] * dt;
     double lam = 0.0;
     if (kind == ST_RHS) {
         if (valid)

@@ -97,8 +97,8 @@ def kernel_bytes_per_elem(N, n_p, s, kernel):
     and u_n and writes u4; stage 4 reads u4 and W and writes u_n+1: 2, 4,
     3, 3 arrays).  Matrix-core path (kernel 2/3): the packed geometry, per
     face one record of 4 values (nx, ny, len, neighbour word), per element a
-    map of 4 values (xr, yr, xs, ys) and J, 17*s bytes (68 in FP32).
-    Lane-per-element path (kernel 1): geometry 14*s (xr, yr, xs, ys, J; per
+    map of 4 values (xr, yr, xs, ys) and 1/J, 17*s bytes (68 in FP32).
+    Lane-per-element path (kernel 1): geometry 14*s (xr, yr, xs, ys, 1/J; per
     face nx, ny, len) and 3 32-bit neighbour words."""
     if kernel >= 2:
         return 3.0 * N * n_p * s + (3 * 4 + 4 + 1) * s
