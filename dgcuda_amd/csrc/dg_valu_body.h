@@ -59,6 +59,22 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
         for (int m = 0; m < NP; m++) tot[v][m] = R(0);
 
+    // The neighbour words and the first face's neighbour modes are requested
+    // before the volume integral, whose arithmetic covers their latency (the
+    // kernel waits on its loads for about half of its time; 1-11% faster at
+    // p = 1, 2).  The BND variant keeps its loads in the face loop.
+    int wf[3];
+    R nu[N][NP];
+    if constexpr (!BND) {
+#pragma unroll
+        for (int f = 0; f < 3; f++) wf[f] = gp(a.nbr)[f * Kp + ee];
+        const int nb0 = wf[0] & NB_ELEM_MASK;
+#pragma unroll
+        for (int v = 0; v < N; v++)
+#pragma unroll
+            for (int m = 0; m < NP; m++) nu[v][m] = gp(a.u)[(size_t)(v * NP + m) * Kp + nb0];
+    }
+
     // ---- volume: sum_q F(U_q) . (metric * grad phi_m) ----
     {
         const R xr = gp(a.geo_e)[ee], yr = gp(a.geo_e)[Kp + ee], xs = gp(a.geo_e)[2 * Kp + ee], ys = gp(a.geo_e)[3 * Kp + ee];
@@ -98,7 +114,19 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
     // ---- the three faces, added in face order as eval_rhs adds s1, s2, s3 ----
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        const int w = gp(a.nbr)[f * Kp + ee];
+        int w;
+        if constexpr (BND) {
+            w = gp(a.nbr)[f * Kp + ee];
+        } else {
+            w = wf[f];
+            if (f > 0) {
+                const int nbf = w & NB_ELEM_MASK;
+#pragma unroll
+                for (int v = 0; v < N; v++)
+#pragma unroll
+                    for (int m = 0; m < NP; m++) nu[v][m] = gp(a.u)[(size_t)(v * NP + m) * Kp + nbf];
+            }
+        }
         // boundary side (BND variant): no neighbour, a ghost state
         bool bnd = false;
         int nb = w & NB_ELEM_MASK;
@@ -132,7 +160,10 @@ __device__ __forceinline__ double valu_process(const StageArgs<R>& a, int e, int
 #pragma unroll
                 for (int v = 0; v < N; v++) {
                     Uo[v] += u[v][m] * so;
-                    Un[v] += gp(a.u)[(size_t)(v * NP + m) * Kp + nb] * sn;
+                    if constexpr (BND)
+                        Un[v] += gp(a.u)[(size_t)(v * NP + m) * Kp + nb] * sn;
+                    else
+                        Un[v] += nu[v][m] * sn;
                 }
             }
             if constexpr (BND)

@@ -60,7 +60,7 @@ static int run(const Mesh& m) {
         ge[Kp + i] = (R)g.yr[e];
         ge[2 * Kp + i] = (R)g.xs[e];
         ge[3 * Kp + i] = (R)g.ys[e];
-        ge[4 * Kp + i] = (R)g.J[e];
+        ge[4 * Kp + i] = (R)(1.0 / g.J[e]);  // the inverse mass, as the solver stores it
         for (int f = 0; f < 3; f++) {
             gf[(3 * f) * Kp + i] = (R)g.fnx[f][e];
             gf[(3 * f + 1) * Kp + i] = (R)g.fny[f][e];

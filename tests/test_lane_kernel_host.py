@@ -44,3 +44,17 @@ def test_lane_body_accesses_in_bounds(harness, mesh, N, p, prec):
                        stderr=subprocess.STDOUT, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-3000:]
     assert "every access in bounds" in r.stdout
+
+
+@pytest.mark.parametrize("N,p,prec", [(4, 1, 1), (4, 2, 1), (4, 2, 0), (1, 2, 1)])
+def test_lane_body_accesses_in_bounds_periodic(harness, tmp_path, N, p, prec):
+    """The variant without boundary code (periodic meshes), whose neighbour
+    words and first neighbour's modes are loaded ahead of the volume
+    integral, on a jittered periodic mesh written as .pmsh."""
+    import dgcuda_amd as dg
+    path = str(tmp_path / "periodic.pmsh")
+    dg.Mesh.periodic_jittered(12, 10, 0.0, 10.0, 0.0, 10.0, 0.1, 3).write_pmsh(path)
+    r = subprocess.run([harness, path, str(N), str(p), str(prec)], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert "every access in bounds" in r.stdout and "boundary faces 0" in r.stdout
