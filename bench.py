@@ -155,7 +155,7 @@ def batch_summary(copy_gbs, steps=20, steps64=4):
                                                  speedup=round(el2 / elc, 3)),
                          bounds=bounds(detail, el2, copy_gbs))
     out["kernel_choice"] = ("fixed per order: lane items (one element per lane) for p <= 2, matrix-core tiles "
-                            "for p >= 3 (the autotuner's choice for single FP32 solves at large sizes; for FP64 "
+                            "for p >= 3 (the autotuner's choice for single FP32 solves at large sizes; for FP64 Euler "
                             "p = 2 it picks tiles, which batched measured no faster)")
     out["launches"] = ("one persistent launch per RK stage (speedup); per_class_launches: one per stage and "
                        "kernel class (Euler/advection x tiles/lane items), each at its own occupancy")
