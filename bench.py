@@ -130,7 +130,7 @@ def bounds(detail, elapsed, copy_gbs):
 
 def batch_summary(copy_gbs, steps=20, steps64=4):
     """Ragged batch: 64 solves (500-200k triangles, p 1..4, half advection
-    and half Euler vortices) in one persistent launch per RK stage, against
+    and half Euler vortices) in each launch form of the batch, against
     the same solves one after another on the single-solve path, and the
     bounds of any batching of them."""
     import dgcuda_amd as dg
@@ -139,26 +139,30 @@ def batch_summary(copy_gbs, steps=20, steps64=4):
     cases = W.ragged_batch_c4()
     out = dict(workload="ragged_batch_64_solves_500_to_200k_p1to4", solves=len(cases),
                triangles=sum(m.num_elem for m, _, _ in cases))
+    forms = ((dg.DG_BATCH_ONE_LAUNCH, "one_launch"), (dg.DG_BATCH_PER_CLASS, "per_class"))
     for prec, name, n in ((dg.DG_FP32, "fp32", steps), (dg.DG_FP64, "fp64", steps64)):
-        el, work, b = W.run_batch(cases, n, prec)
-        b.close()
-        elc, _, b = W.run_batch(cases, n, prec, launch_mode=dg.DG_BATCH_PER_CLASS)
-        b.close()
+        el = {}
+        for mode, fname in forms:
+            el[mode], work, b = W.run_batch(cases, n, prec, launch_mode=mode)
+            b.close()
         detail = {}
         el2, _, ss = W.run_sequential(cases, n, prec, detail=detail)
         for x in ss:
             x.close()
-        out[name] = dict(steps=n, batched_dof_stage_per_s=float("%.4g" % (work / el)),
+        dflt = dg.BATCH_DEFAULT_LAUNCH_MODE[prec]
+        out[name] = dict(steps=n, launch_form=dict(forms)[dflt],
+                         batched_dof_stage_per_s=float("%.4g" % (work / el[dflt])),
                          one_at_a_time_dof_stage_per_s=float("%.4g" % (work / el2)),
-                         speedup=round(el2 / el, 3),
-                         per_class_launches=dict(dof_stage_per_s=float("%.4g" % (work / elc)),
-                                                 speedup=round(el2 / elc, 3)),
+                         speedup=round(el2 / el[dflt], 3),
+                         forms={fname: dict(dof_stage_per_s=float("%.4g" % (work / el[mode])),
+                                            speedup=round(el2 / el[mode], 3)) for mode, fname in forms},
                          bounds=bounds(detail, el2, copy_gbs))
     out["kernel_choice"] = ("fixed per order: lane items (one element per lane) for p <= 2, matrix-core tiles "
                             "for p >= 3 (the autotuner's choice for single FP32 solves at large sizes; for FP64 Euler "
                             "p = 2 it picks tiles, which batched measured no faster)")
-    out["launches"] = ("one persistent launch per RK stage (speedup); per_class_launches: one per stage and "
-                       "kernel class (Euler/advection x tiles/lane items), each at its own occupancy")
+    out["launches"] = ("forms: one_launch = one persistent launch per RK stage carrying every kernel class; "
+                       "per_class = one per stage and kernel class (Euler/advection x tiles/lane items), each at "
+                       "its own occupancy; speedup is the batch's default form (launch_form) in each precision")
     return out
 
 

@@ -32,6 +32,8 @@ DG_FLUX_LLF, DG_FLUX_CENTRAL = 0, 1
 DG_QUIRK_FLOAT_SQRT, DG_QUIRK_FLOAT_GEOM, DG_QUIRK_FLOAT_POW, DG_QUIRK_REFLECT_MIRROR = 1, 2, 4, 8
 DG_GMSH_QUIRKS, DG_GMSH_PERIODIC = 1, 2
 DG_BATCH_ONE_LAUNCH, DG_BATCH_PER_CLASS = 0, 1
+# the launch form a new batch starts with, per precision (the library's default)
+BATCH_DEFAULT_LAUNCH_MODE = {DG_FP32: DG_BATCH_ONE_LAUNCH, DG_FP64: DG_BATCH_PER_CLASS}
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_IO", 3: "DG_ERR_PARSE", 4: "DG_ERR_HIP",
                 5: "DG_ERR_UNSUPPORTED", 6: "DG_ERR_NUMERIC", 7: "DG_ERR_BUDGET", 8: "DG_ERR_STATE"}
 STATUS_CODES = {v: k for k, v in STATUS_NAMES.items()}
@@ -513,9 +515,9 @@ class Batch:
         _check(lib().dg_batch_set_max_groups(self._h, n))
 
     def set_launch_mode(self, mode):
-        """DG_BATCH_ONE_LAUNCH (default: one persistent launch per stage) or
-        DG_BATCH_PER_CLASS (one per kernel class and stage); from the next
-        run on."""
+        """DG_BATCH_ONE_LAUNCH (one persistent launch per stage; the FP32
+        default) or DG_BATCH_PER_CLASS (one per kernel class and stage; the
+        FP64 default); from the next run on."""
         _check(lib().dg_batch_set_launch_mode(self._h, mode))
 
     def launch_info(self):

@@ -1,9 +1,11 @@
 // Many independent solves of uneven size and mixed order advanced together:
 // persistent launches per RK stage over device-side work lists of (solve,
 // tile) items, the step (per-solve dt kernel + the stage launches) replayed
-// from a hipGraph.  By default each kernel class present (Euler tiles,
-// advection tiles, Euler lane items, advection lane items) is one launch per
-// stage; the one-launch form runs every class in a single kernel per stage.
+// from a hipGraph.  Two launch forms: one kernel per stage carrying every
+// class (the FP32 default), or one launch per stage for each kernel class
+// present (Euler tiles, advection tiles, Euler lane items, advection lane
+// items), each at the occupancy its own bodies allow (the FP64 default: the
+// all-class FP64 kernel holds one wavefront per SIMD).
 //
 // Every solve keeps its own buffers, geometry and control block (its own
 // CFL dt, time and end time), so a solve's arithmetic does not depend on
@@ -391,7 +393,10 @@ class EngineT : public BatchEngine {
     int cap_slots = 0, cap_items = 0;
     int nslots = 0, nitems = 0, dt_grid = 0, cus = 0;
     int max_groups_req = 0;          // set_max_groups (0: no cap)
-    int form = DG_BATCH_ONE_LAUNCH;  // set_launch_form
+    // set_launch_form; the default is the form measured faster on the
+    // 64-solve ragged mix in each precision (FP32 one launch: 1.15x the
+    // per-class form; FP64 per class: 1.12x the one launch)
+    int form = sizeof(R) == 8 ? DG_BATCH_PER_CLASS : DG_BATCH_ONE_LAUNCH;
     std::vector<Launch> launches;    // the batch kernels of a stage, in launch order
     std::vector<int2> raw_items[BATCH_LISTS];  // the work lists in solve order, and
     std::vector<double> raw_cost[BATCH_LISTS]; // each item's estimated cost

@@ -131,8 +131,8 @@ def _ceiling(runs, elapsed, detail):
 
 def run_batch(cases, steps, precision=DG_FP32, use_graph=True, warmup=2, launch_mode=None):
     """All cases in one batch, `steps` RK4 steps each; returns (seconds,
-    DOF*stage, batch).  launch_mode: DG_BATCH_ONE_LAUNCH (the default) or
-    DG_BATCH_PER_CLASS."""
+    DOF*stage, batch).  launch_mode: DG_BATCH_ONE_LAUNCH or
+    DG_BATCH_PER_CLASS (None: the batch's default for the precision)."""
     b = Batch(precision)
     if launch_mode is not None:
         b.set_launch_mode(launch_mode)

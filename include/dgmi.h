@@ -354,15 +354,15 @@ int dg_batch_info(const dg_batch* b, int* n_solves, int64_t* total_elems, int64_
  * not depend on it.  Test and tuning aid; no counterpart in the reference. */
 int dg_batch_set_max_groups(dg_batch* b, int max_groups);
 /* How a stage of the batch is launched; takes effect at the next run.
- * DG_BATCH_ONE_LAUNCH (default): a single persistent launch per stage
- * carrying every body, at the register budget of the largest; workgroups
- * mix light and heavy items.  DG_BATCH_PER_CLASS: one persistent launch per
- * kernel class present (Euler tiles, advection tiles, Euler lane items,
- * advection lane items), each compiled with its own bodies only and so
- * holding the wavefronts per SIMD those bodies allow (faster in FP64, whose
- * one-launch kernel holds one wavefront per SIMD).  Results are the same
- * bit for bit.  No counterpart in the reference (its solves run one at a
- * time). */
+ * DG_BATCH_ONE_LAUNCH (the FP32 default): a single persistent launch per
+ * stage carrying every body, at the register budget of the largest;
+ * workgroups mix light and heavy items.  DG_BATCH_PER_CLASS (the FP64
+ * default): one persistent launch per kernel class present (Euler tiles,
+ * advection tiles, Euler lane items, advection lane items), each compiled
+ * with its own bodies only and so holding the wavefronts per SIMD those
+ * bodies allow (the FP64 one-launch kernel holds one wavefront per SIMD).
+ * Results are the same bit for bit.  No counterpart in the reference (its
+ * solves run one at a time). */
 enum { DG_BATCH_ONE_LAUNCH = 0, DG_BATCH_PER_CLASS = 1 };
 int dg_batch_set_launch_mode(dg_batch* b, int mode);
 /* Launch shape of the last uploaded work list (uploaded by the first run
