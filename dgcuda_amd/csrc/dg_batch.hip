@@ -836,6 +836,7 @@ class BatchImpl : public Batch {
         std::vector<Ctrl> h;
         Status st = eng->pull_ctrls(h);
         if (!st) return st;
+        if (h.size() != solvers.size()) return Status::err(DG_ERR_STATE, "batch slots out of date");
         for (size_t s = 0; s < solvers.size(); s++) {
             solvers[s]->set_host_ctrl(h[s]);
             solvers[s]->set_lambda_valid(true);

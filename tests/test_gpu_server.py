@@ -27,9 +27,9 @@ def requests(n, seed=3):
     return out
 
 
-def alone(req, solo_min_elems):
+def alone(req, solo_min_elems, prec=dg.DG_FP32):
     """The same request served by itself."""
-    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems,
+    sv = dg.Server(prec, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems,
                    fifo_min_elems=0)
     sv.keep_states(True)
     r = dg.request(req.nx, req.p, req.t_end, req.prob, mesh_seed=req.mesh_seed, arrival_s=0.0)
@@ -42,17 +42,19 @@ def alone(req, solo_min_elems):
     return state
 
 
-@pytest.mark.parametrize("solo_min_elems,fifo_min_elems", [(0, 0), (400, 0), (400, 600)])
-def test_server_results_equal_alone(solo_min_elems, fifo_min_elems):
+@pytest.mark.parametrize("solo_min_elems,fifo_min_elems,prec",
+                         [(0, 0, dg.DG_FP32), (400, 0, dg.DG_FP32), (400, 600, dg.DG_FP32), (400, 600, dg.DG_FP64)])
+def test_server_results_equal_alone(solo_min_elems, fifo_min_elems, prec):
     """In the mix, every result equals the request served alone; with a
     threshold of 400 elements the larger requests run their own kernels
     inside the step graph while the others share the batch launch; with a
     FIFO threshold of 600 the requests of 600 elements and more advance one
-    at a time in arrival order, the others beside them."""
+    at a time in arrival order, the others beside them.  In FP64 the batch
+    launch is one kernel per class (Euler tiles, lane items, ...)."""
     reqs = requests(10)
     assert any(2 * r.nx * r.ny >= 400 for r in reqs) and any(2 * r.nx * r.ny < 400 for r in reqs)
     assert sum(2 * r.nx * r.ny >= 600 for r in reqs) >= 2
-    sv = dg.Server(dg.DG_FP32, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems,
+    sv = dg.Server(prec, max_slots=8, steps_per_round=4, cfl=0.3, solo_min_elems=solo_min_elems,
                    fifo_min_elems=fifo_min_elems)
     sv.keep_states(True)
     t0 = sv.now()
@@ -69,7 +71,7 @@ def test_server_results_equal_alone(solo_min_elems, fifo_min_elems):
         assert out.status == 0 and abs(out.t - r.t_end) == 0.0
         shape = (4, dg.n_modes(r.p), 2 * r.nx * r.ny)
         got = sv.get_state(ids[k], shape)
-        assert np.array_equal(got, alone(r, solo_min_elems)), k
+        assert np.array_equal(got, alone(r, solo_min_elems, prec)), k
     for out in res.values():
         assert out.finish_s >= out.arrival_s
         m0, m1 = np.array(out.mass0), np.array(out.mass)
