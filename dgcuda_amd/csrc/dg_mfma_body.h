@@ -45,6 +45,11 @@ __device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
 // FP64 product D[g + 4r][j]; pos(g, r) names that row.  Every reduction is
 // ordered the same way (product s of lane group g takes index pos(g, s)),
 // so a lane's B operands are the rows it holds in the output.
+// Volume and face points are placed by the A tables of the interpolations:
+// row i holds point qrow(i), so register r of lane group g holds point
+// qpos(g, r) = g + 4r in both precisions.  Product step r of the
+// projections then takes points 4r..4r+3, and a step whose points are all
+// padding (4r >= the point count, p <= 3) is left out at compile time.
 template <typename R>
 struct MfmaT;
 template <>
@@ -52,6 +57,8 @@ struct MfmaT<float> {
     using V = floatx4;
     __device__ __forceinline__ static V mma(float a, float b, V c) { return mfma4(a, b, c); }
     __host__ __device__ static constexpr int pos(int g, int r) { return 4 * g + r; }
+    __host__ __device__ static constexpr int qpos(int g, int r) { return g + 4 * r; }
+    __host__ __device__ static constexpr int qrow(int i) { return (i >> 2) + 4 * (i & 3); }
 };
 template <>
 struct MfmaT<double> {
@@ -60,6 +67,8 @@ struct MfmaT<double> {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
     __host__ __device__ static constexpr int pos(int g, int r) { return g + 4 * r; }
+    __host__ __device__ static constexpr int qpos(int g, int r) { return g + 4 * r; }
+    __host__ __device__ static constexpr int qrow(int i) { return i; }
 };
 
 constexpr int MFMA_TILE = 64;
@@ -131,12 +140,13 @@ struct MfmaA {
     __device__ __forceinline__ void init(int g, int j, int z = 0) {
         constexpr int NP = n_modes(P), NQ = n_quad_vol(P), NQ1 = P + 1, NPT = 3 * NQ1;
         const TabP<R, P>& T = *reinterpret_cast<const TabP<R, P>*>(reinterpret_cast<const R*>(&tab<R, P>()) + z);
+        const int qj = MfmaT<R>::qrow(j);  // the point of A row j
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             const int m = MfmaT<R>::pos(g, s);
-            vol[s] = (m < NP && j < NQ) ? T.B[m][j] : R(0);
-            tr[s] = (m < NP && j < NPT) ? T.S[j / NQ1][m][j % NQ1] : R(0);
-            const int q = MfmaT<R>::pos(g, s);
+            vol[s] = (m < NP && qj < NQ) ? T.B[m][qj] : R(0);
+            tr[s] = (m < NP && qj < NPT) ? T.S[qj / NQ1][m][qj % NQ1] : R(0);
+            const int q = MfmaT<R>::qpos(g, s);
             gx[s] = (j < NP && q < NQ) ? T.Gx[j][q] : R(0);
             gy[s] = (j < NP && q < NQ) ? T.Gy[j][q] : R(0);
             sp[s] = (j < NP && q < NPT) ? T.S[q / NQ1][j][q % NQ1] * T.W1[q % NQ1] : R(0);
@@ -358,7 +368,7 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
     const R xr = gc.xr, yr = gc.yr, xs = gc.xs, ys = gc.ys;
 
     // ---- phase 1: face traces of the own sub-tile and of this wave's halo pairs ----
-    // (points pos(g, 0..3) of every lane are written: a slot holds 16 points
+    // (points qpos(g, 0..3) of every lane are written: a slot holds 16 points
     // plus a padding row; points >= 3(p+1) are read by the surface
     // projection only to be replaced by zero)
     const V Atr = tabA[MFMA_A_TR * 64 + lane];
@@ -372,7 +382,7 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            R* dst = ts + el * TS + M::pos(g, i) * N;
+            R* dst = ts + el * TS + M::qpos(g, i) * N;
             if constexpr (N == 4)
                 *reinterpret_cast<V*>(dst) = V{T[0][i], T[1][i], T[2][i], T[3][i]};
             else
@@ -388,7 +398,7 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
 #pragma unroll
             for (int s = 0; s < 4; s++) acc = M::mma(Atr[s], st.hb[q][s], acc);
 #pragma unroll
-            for (int i = 0; i < 4; i++) ts[slot * TS + M::pos(g, i) * N + v] = acc[i];
+            for (int i = 0; i < 4; i++) ts[slot * TS + M::qpos(g, i) * N + v] = acc[i];
         }
     }
     // face records, and the side list: the thread of record t = 3 el + f of
@@ -459,11 +469,13 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
     const V Agx = tabA[MFMA_A_GX * 64 + lane], Agy = tabA[MFMA_A_GY * 64 + lane];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        // volume point q = pos(g, i): flux, metric, reduction step i
+        // volume point q = qpos(g, i): flux, metric, reduction step i (steps
+        // of padding points only are left out)
+        if (4 * i >= NQ) continue;
         R Uq[N], fx[N], fy[N];
 #pragma unroll
         for (int v = 0; v < N; v++) Uq[v] = U[v][i];
-        const bool qv = M::pos(g, i) < NQ;
+        const bool qv = M::qpos(g, i) < NQ;
         if (N == 4 && !qv) Uq[0] = R(1);  // padding point: keep the flux finite
         if constexpr (N == 4) {
             metric_flux<R>(a.phys, Uq, xr, yr, xs, ys, fx, fy);  // fx, fy hold Fr, Fs
@@ -521,9 +533,11 @@ __device__ __forceinline__ double mfma_process_tile_pipe(const StageArgs<R>& a, 
 
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        // own face point pt = pos(g, i): its flux from phase 2b (point slots
-        // past 3(p+1) hold stale traces and meet zero)
-        const int pt = M::pos(g, i);
+        // own face point pt = qpos(g, i): its flux from phase 2b (point slots
+        // past 3(p+1) hold stale traces and meet zero; steps of such slots
+        // only are left out)
+        if (4 * i >= NPT) continue;
+        const int pt = M::qpos(g, i);
         const R* gq = ts + el * TS + pt * N;
         R G[N];
         if constexpr (N == 4) {
